@@ -1,0 +1,355 @@
+"""freeshearlayer_amd -- MI355X-native per-step update of the freeShearLayer
+compressible LES solver.
+
+The product is the native library ``lib/libfsl.so`` (hand-written HIP kernels
+for gfx950 behind the C-ABI of ``include/fsl.h``, plus the ANSI-C host layer).
+This module is a thin ctypes binding over that C-ABI used by the tests and by
+``bench.py``.  There is no Python or CPU fallback: if the library is missing,
+``load()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libfsl.so")
+LAYER2_PATH = os.path.join(LIB_DIR, "libfsl_layer2.so")
+
+FSL_OK = 0
+FSL_SGS_CONSTANT, FSL_SGS_DYNAMIC, FSL_SGS_LEARNED = 0, 1, 2
+FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
+FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class FslCase(C.Structure):
+    """The 22 positional items of layer2.ini (include/fsl.h: fsl_case)."""
+    _fields_ = [("LEN", C.c_uint), ("HIG", C.c_uint), ("DEP", C.c_uint),
+                ("deltaX", C.c_double), ("deltaY", C.c_double), ("deltaZ", C.c_double),
+                ("deltaT", C.c_double), ("Answer", C.c_uint), ("numstep", C.c_uint),
+                ("mu_L", C.c_double), ("Pr_L", C.c_double), ("Pr_T", C.c_double),
+                ("Cs", C.c_double), ("BL_HIG", C.c_uint), ("Ua", C.c_double), ("Va", C.c_double),
+                ("Wa", C.c_double), ("Ns_min", C.c_int), ("Nst", C.c_int), ("f_step", C.c_uint),
+                ("nStages", C.c_int), ("maxCoNum", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FslConfig(C.Structure):
+    _fields_ = [("sgs_model", C.c_int), ("precision", C.c_int), ("cfl_mode", C.c_int),
+                ("cfl", C.c_double), ("z_quirk", C.c_int), ("fused", C.c_int),
+                ("rho_in", C.c_double), ("p_in", C.c_double), ("p_out", C.c_double),
+                ("device", C.c_int)]
+
+
+class FslBlock(C.Structure):
+    _fields_ = [("X", C.c_double), ("counter", C.c_int), ("k_min", C.c_uint), ("k_max", C.c_uint),
+                ("Ud", C.c_double), ("Vd", C.c_double), ("Wd", C.c_double)]
+
+
+class FslShearCase(C.Structure):
+    _fields_ = [("mach_c", C.c_double), ("ratio", C.c_double), ("delta_w0", C.c_double),
+                ("reynolds", C.c_double), ("pert", C.c_double), ("rho0", C.c_double),
+                ("p0", C.c_double), ("seed", C.c_ulonglong)]
+
+
+_lib = None
+
+
+def load():
+    """Load lib/libfsl.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("freeshearlayer_amd: native library %s is missing; run "
+                           "`python -m freeshearlayer_amd.build` (no fallback exists)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i_, d_, u_, cc = C.c_void_p, C.c_int, C.c_double, C.c_uint, C.c_char_p
+    sig = {
+        "fsl_case_read_ini": (i_, [cc, C.POINTER(FslCase)]),
+        "fsl_case_read_bin": (i_, [cc, C.POINTER(FslCase)]),
+        "fsl_case_ini_to_bin": (i_, [cc, cc]),
+        "fsl_case_validate": (i_, [C.POINTER(FslCase), C.c_char_p, C.c_size_t]),
+        "fsl_random": (d_, [d_]),
+        "fsl_minmod": (d_, [d_, d_]),
+        "fsl_block_init": (None, [C.POINTER(FslBlock), d_]),
+        "fsl_block_advance": (i_, [C.POINTER(FslBlock), C.POINTER(FslCase)]),
+        "fsl_ic_reference": (None, [C.POINTER(FslCase), _dp]),
+        "fsl_shear_defaults": (None, [C.POINTER(FslShearCase)]),
+        "fsl_ic_shear_layer": (d_, [C.POINTER(FslCase), C.POINTER(FslShearCase), _dp, _dp]),
+        "fsl_backup_write": (i_, [cc, C.POINTER(FslCase), _dp, d_, u_, i_]),
+        "fsl_backup_read": (i_, [cc, C.POINTER(FslCase), _dp, _dp, C.POINTER(u_), i_, i_]),
+        "fsl_config_defaults": (None, [C.POINTER(FslConfig)]),
+        "fsl_create": (i_, [C.POINTER(FslCase), C.POINTER(FslConfig), C.POINTER(vp)]),
+        "fsl_destroy": (None, [vp]),
+        "fsl_last_error": (cc, [vp]),
+        "fsl_ncell": (C.c_size_t, [vp]),
+        "fsl_nface": (C.c_size_t, [vp, i_]),
+        "fsl_device_bytes": (C.c_size_t, [vp]),
+        "fsl_set_inflow": (i_, [vp, _dp]),
+        "fsl_upload_state": (i_, [vp, i_, _dp]),
+        "fsl_download_state": (i_, [vp, i_, _dp]),
+        "fsl_download_face": (i_, [vp, i_, _dp]),
+        "fsl_upload_face": (i_, [vp, i_, _dp]),
+        "fsl_download_mu_sgs": (i_, [vp, _dp]),
+        "fsl_upload_mu_sgs": (i_, [vp, _dp]),
+        "fsl_set_dt": (i_, [vp, d_]),
+        "fsl_get_dt": (d_, [vp]),
+        "fsl_set_block": (i_, [vp, u_, u_, d_, d_, d_]),
+        "fsl_get_cur": (i_, [vp]),
+        "fsl_set_cur": (i_, [vp, i_]),
+        "fsl_ghost_fill": (i_, [vp]),
+        "fsl_reconstruct": (i_, [vp]),
+        "fsl_iface_bc": (i_, [vp]),
+        "fsl_fluxes": (i_, [vp]),
+        "fsl_evolution": (i_, [vp, i_, i_]),
+        "fsl_rk_update": (i_, [vp, i_, i_]),
+        "fsl_check_co_num": (i_, [vp, _dp]),
+        "fsl_dynamic_smagorinsky": (i_, [vp]),
+        "fsl_filter": (i_, [vp, _dp, _dp, _dp]),
+        "fsl_stage": (i_, [vp, i_]),
+        "fsl_step": (i_, [vp]),
+        "fsl_run": (i_, [vp, i_]),
+        "fsl_set_lcg": (i_, [vp, d_, i_]),
+        "fsl_get_lcg": (i_, [vp, _dp, C.POINTER(i_), C.POINTER(u_), C.POINTER(u_), _dp, _dp, _dp]),
+        "fsl_get_step": (u_, [vp]),
+        "fsl_set_step": (i_, [vp, u_]),
+        "fsl_get_block_state": (i_, [vp, C.POINTER(FslBlock)]),
+        "fsl_set_block_state": (i_, [vp, C.POINTER(FslBlock)]),
+        "fsl_get_case": (i_, [vp, C.POINTER(FslCase)]),
+        "fsl_get_config": (i_, [vp, C.POINTER(FslConfig)]),
+        "fsl_synchronize": (i_, [vp]),
+        "fsl_conservation_sums": (i_, [vp, _dp]),
+        "fsl_profile_enable": (i_, [vp, i_]),
+        "fsl_profile_reset": (i_, [vp]),
+        "fsl_profile_read": (i_, [vp, i_, _dp, C.POINTER(C.c_longlong)]),
+        "fsl_kernel_name": (cc, [i_]),
+        "fsl_kernel_count": (i_, []),
+        "fsl_copy_kernel": (i_, [vp, C.c_size_t, i_]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI functions this binding expects libfsl.so to export."""
+    import re
+    hdr = os.path.join(os.path.dirname(PKG_DIR), "include", "fsl.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(fsl_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _ptr(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def case_reference(L, H, D, **over):
+    """Case items of the reference layer2.ini with the mesh replaced."""
+    c = FslCase(LEN=L, HIG=H, DEP=D, deltaX=1.6666667, deltaY=1.0, deltaZ=1.5, deltaT=0.0007,
+                Answer=0, numstep=3000, mu_L=0.253358, Pr_L=0.72, Pr_T=0.8, Cs=0.01, BL_HIG=5,
+                Ua=35.0, Va=35.0, Wa=35.0, Ns_min=10, Nst=50, f_step=50, nStages=3, maxCoNum=0.1)
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c
+
+
+def config(**over):
+    cfg = FslConfig()
+    load().fsl_config_defaults(C.byref(cfg))
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class FslError(RuntimeError):
+    pass
+
+
+class Context:
+    """Device context over the C-ABI (one HIP stream, device-resident state)."""
+
+    def __init__(self, case, cfg=None):
+        self.lib = load()
+        self.case = case
+        self.cfg = cfg if cfg is not None else config()
+        h = C.c_void_p()
+        rc = self.lib.fsl_create(C.byref(self.case), C.byref(self.cfg), C.byref(h))
+        if rc != FSL_OK:
+            raise FslError("fsl_create failed (%d)" % rc)
+        self.h = h
+        self.L, self.H, self.D = case.LEN, case.HIG, case.DEP
+        self.cell_shape = (self.L + 2, self.H + 2, self.D + 2)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fsl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != FSL_OK:
+            raise FslError("%s failed (%d): %s" % (what, rc, self.lib.fsl_last_error(self.h).decode()))
+        return rc
+
+    # --- transfers ---
+    def upload_state(self, U, which=0):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape((5,) + self.cell_shape)
+        self._chk(self.lib.fsl_upload_state(self.h, which, _ptr(U)), "upload_state")
+
+    def download_state(self, which=0):
+        U = np.zeros((5,) + self.cell_shape)
+        self._chk(self.lib.fsl_download_state(self.h, which, _ptr(U)), "download_state")
+        return U
+
+    def face_shape(self, fid):
+        d = fid // 10
+        return (self.L + (d == 0), self.H + (d == 1), self.D + (d == 2))
+
+    def download_face(self, fid):
+        out = np.zeros(self.face_shape(fid))
+        self._chk(self.lib.fsl_download_face(self.h, fid, _ptr(out)), "download_face")
+        return out
+
+    def upload_face(self, fid, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.lib.fsl_upload_face(self.h, fid, _ptr(a)), "upload_face")
+
+    def download_mu(self):
+        out = np.zeros(self.cell_shape)
+        self._chk(self.lib.fsl_download_mu_sgs(self.h, _ptr(out)), "download_mu_sgs")
+        return out
+
+    def upload_mu(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.lib.fsl_upload_mu_sgs(self.h, _ptr(a)), "upload_mu_sgs")
+
+    def set_inflow(self, u_in):
+        u = np.ascontiguousarray(u_in, dtype=np.float64)
+        self._chk(self.lib.fsl_set_inflow(self.h, _ptr(u)), "set_inflow")
+
+    # --- scalars ---
+    def set_dt(self, dt):
+        self._chk(self.lib.fsl_set_dt(self.h, dt), "set_dt")
+
+    @property
+    def dt(self):
+        return self.lib.fsl_get_dt(self.h)
+
+    def set_block(self, kmin, kmax, ud, vd, wd):
+        self._chk(self.lib.fsl_set_block(self.h, kmin, kmax, ud, vd, wd), "set_block")
+
+    def set_lcg(self, X, counter=0):
+        self._chk(self.lib.fsl_set_lcg(self.h, X, counter), "set_lcg")
+
+    def lcg(self):
+        X = C.c_double(); cnt = C.c_int(); kmin = C.c_uint(); kmax = C.c_uint()
+        ud = C.c_double(); vd = C.c_double(); wd = C.c_double()
+        self.lib.fsl_get_lcg(self.h, C.byref(X), C.byref(cnt), C.byref(kmin), C.byref(kmax),
+                             C.byref(ud), C.byref(vd), C.byref(wd))
+        return dict(X=X.value, counter=cnt.value, k_min=kmin.value, k_max=kmax.value,
+                    Ud=ud.value, Vd=vd.value, Wd=wd.value)
+
+    @property
+    def cur_is_p(self):
+        return self.lib.fsl_get_cur(self.h)
+
+    def set_cur(self, is_p):
+        self._chk(self.lib.fsl_set_cur(self.h, int(is_p)), "set_cur")
+
+    @property
+    def step_count(self):
+        return self.lib.fsl_get_step(self.h)
+
+    # --- operations (reference entry points) ---
+    def ghost_fill(self): self._chk(self.lib.fsl_ghost_fill(self.h), "ghost_fill")
+    def reconstruct(self): self._chk(self.lib.fsl_reconstruct(self.h), "reconstruct")
+    def iface_bc(self): self._chk(self.lib.fsl_iface_bc(self.h), "iface_bc")
+    def fluxes(self): self._chk(self.lib.fsl_fluxes(self.h), "fluxes")
+    def evolution(self, n, s): self._chk(self.lib.fsl_evolution(self.h, n, s), "evolution")
+    def rk_update(self, scheme, s): self._chk(self.lib.fsl_rk_update(self.h, scheme, s), "rk_update")
+    def dynamic_smagorinsky(self): self._chk(self.lib.fsl_dynamic_smagorinsky(self.h), "dynamic")
+    def stage(self, s): self._chk(self.lib.fsl_stage(self.h, s), "stage")
+    def step(self): self._chk(self.lib.fsl_step(self.h), "step")
+    def run(self, n): self._chk(self.lib.fsl_run(self.h, n), "run")
+    def synchronize(self): self._chk(self.lib.fsl_synchronize(self.h), "synchronize")
+
+    def check_co_num(self):
+        m = C.c_double()
+        self._chk(self.lib.fsl_check_co_num(self.h, C.byref(m)), "check_co_num")
+        return m.value
+
+    def filter(self, a, h=(1. / 3., 1. / 3., 1. / 3.)):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.cell_shape)
+        out = np.zeros(self.cell_shape)
+        hh = np.array(h, dtype=np.float64)
+        self._chk(self.lib.fsl_filter(self.h, _ptr(a), _ptr(out), _ptr(hh)), "filter")
+        return out
+
+    def conservation_sums(self):
+        out = np.zeros(5)
+        self._chk(self.lib.fsl_conservation_sums(self.h, _ptr(out)), "sums")
+        return out
+
+    # --- measurement ---
+    def profile(self, on=True):
+        self.lib.fsl_profile_enable(self.h, int(on))
+
+    def profile_reset(self):
+        self.lib.fsl_profile_reset(self.h)
+
+    def profile_read(self):
+        n = self.lib.fsl_kernel_count()
+        ms = np.zeros(n)
+        cnt = (C.c_longlong * n)()
+        self.lib.fsl_profile_read(self.h, n, _ptr(ms), cnt)
+        return {self.lib.fsl_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+
+    def copy_kernel(self, n, reps):
+        self._chk(self.lib.fsl_copy_kernel(self.h, n, reps), "copy_kernel")
+
+    @property
+    def device_bytes(self):
+        return self.lib.fsl_device_bytes(self.h)
+
+
+def ic_reference(case):
+    U = np.zeros((5, case.LEN + 2, case.HIG + 2, case.DEP + 2))
+    load().fsl_ic_reference(C.byref(case), _ptr(U))
+    return U
+
+
+def shear_defaults():
+    s = FslShearCase()
+    load().fsl_shear_defaults(C.byref(s))
+    return s
+
+
+def ic_shear_layer(case, shear=None):
+    """Synthetic shear-layer state and inflow profile; returns (U, u_in, mu_L)."""
+    s = shear if shear is not None else shear_defaults()
+    U = np.zeros((5, case.LEN + 2, case.HIG + 2, case.DEP + 2))
+    u_in = np.zeros(case.HIG)
+    mu = load().fsl_ic_shear_layer(C.byref(case), C.byref(s), _ptr(U), _ptr(u_in))
+    return U, u_in, mu

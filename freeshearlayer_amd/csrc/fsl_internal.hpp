@@ -1,0 +1,83 @@
+// fsl_internal.hpp -- launch-level interface between the kernel translation
+// units and the context layer (fsl_api.hip).  Not part of the C-ABI.
+#pragma once
+
+#include "fsl_device.hpp"
+
+namespace fsl {
+
+// The 30 face arrays of the unfused path, in the id order of the C-ABI:
+// 0..4 xU*, 5..9 U*x, 10..14 yU*, 15..19 U*y, 20..24 zU*, 25..29 U*z.
+struct Faces {
+    double* a[30];
+};
+
+// Kernel ids for the event profiler (names in fsl_api.hip)
+enum KernelId {
+    K_GHOST = 0,
+    K_RECON,
+    K_IFACE,
+    K_FLUX_X,
+    K_FLUX_Y,
+    K_FLUX_Z,
+    K_RK,
+    K_CFL,
+    K_SGS_PRIM,
+    K_FILT_Y,
+    K_FILT_Z,
+    K_FILT_X,
+    K_SGS_STRAIN,
+    K_SGS_FINAL,
+    K_FUSED_X,
+    K_FUSED_Y,
+    K_FUSED_Z,
+    K_SUM,
+    K_COPY,
+    K_FUSED_XYZ,
+    K_NKERNELS
+};
+
+// Time-step ratios deltaT_X, deltaT_Y, deltaT_Z
+struct Dt3 {
+    double x, y, z;
+};
+
+// --- unfused path (fsl_kernels.hip) ---
+void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b);
+void launch_recon(hipStream_t s, CState5 U, const Faces& F, const Geo& g, int quirk);
+void launch_iface_bc(hipStream_t s, const Faces& F, const Geo& g, const Bnd& b);
+void launch_flux(hipStream_t s, int dir, CState5 U, const Faces& F, const double* mu, const Geo& g,
+                 const Phys& ph);
+// kind: 31, 32, 33 = three-stage update of stage 1..3; 21, 22 = two-stage
+void launch_rk(hipStream_t s, int kind, State5 U, State5 Up, const Faces& F, const Geo& g, Dt3 dt);
+// max-reduction of the Courant measure into *out (bits of a non-negative double);
+// mode 0: (rho u / rho) dtX, mode 1: (|u|+c)/dx + (|v|+c)/dy + (|w|+c)/dz
+void launch_cfl(hipStream_t s, int mode, CState5 U, const Geo& g, const Phys& ph, double dtX,
+                unsigned long long* out);
+// deterministic FP64 sums of the five fields over the interior
+void launch_sums(hipStream_t s, CState5 U, const Geo& g, double* partial, int nblocks, double* out);
+int sums_nblocks(const Geo& g);
+void launch_copy(hipStream_t s, const double* a, double* b, size_t n);
+
+// --- dynamic Smagorinsky (fsl_sgs.hip) ---
+// Scratch: 9 primitive/product arrays, 9 filtered, 2 filter temporaries per
+// filtered field, L (6), B (6), A (6), A filtered (6), |S|.
+struct SgsBufs {
+    double* prim[9];    // u, v, w, uu, uv, uw, vv, vw, ww
+    double* filt[9];    // their box-filtered fields
+    double* t1[9];      // y-pass temporaries
+    double* t2[9];      // z-pass temporaries
+    double* Lij[6];     // Leonard stress (11, 12, 13, 22, 23, 33)
+    double* Bij[6];     // |S^| S^_ij
+    double* Aij[6];     // |S| S_ij, ghost layer kept 0
+    double* Af[6];      // filtered A
+    double* mag;        // |S|
+};
+size_t sgs_scratch_arrays();   // number of cell arrays SgsBufs needs
+void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
+                void (*prof)(void*, int, int), void* prof_arg);
+// one box filter of a cell array (y, z, x passes)
+void launch_filter(hipStream_t s, const double* in, double* out, double* t1, double* t2, const Geo& g,
+                   const double h[3], void (*prof)(void*, int, int), void* prof_arg);
+
+}  // namespace fsl

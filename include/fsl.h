@@ -1,0 +1,268 @@
+/*
+ * fsl.h -- C-ABI of the MI355X-native freeShearLayer per-step update.
+ *
+ * Plain C types only (pointers, sizes, doubles).  Two layers:
+ *
+ *   1. Case input and host-side helpers (ANSI C, libfsl.so):
+ *      .ini / .bin readers, the inflow LCG and disturbance block, initial
+ *      conditions, backup.dat I/O.
+ *   2. Device context (HIP on gfx950, libfsl.so): device-resident state and
+ *      the per-stage operations of the reference's solver step.  Each
+ *      operation names the reference function it replaces (file:line in
+ *      nikola-m/freeShearLayer).
+ *
+ * The reference-named drop-in entry points (Reconstruction(), Fluxes(),
+ * Evolution(), ...) operating on a bound context are declared in
+ * fsl_layer2.h and live in libfsl_layer2.so.
+ *
+ * Conventions
+ *   - Every fsl_* function that can fail returns an int status (FSL_OK = 0,
+ *     negative on error); fsl_last_error() gives a message.
+ *   - Host arrays use the dense layouts
+ *       cell  [i][j][k]  (LEN+2)(HIG+2)(DEP+2), k fastest, one ghost layer
+ *       x-face [f][j][k] (LEN+1) HIG DEP
+ *       y-face [i][f][k] LEN (HIG+1) DEP
+ *       z-face [i][j][f] LEN HIG (DEP+1)
+ *     A "state" is the five conserved fields rho, rho u, rho v, rho w, E
+ *     stored one after the other (5 cell arrays).
+ *   - A context is single-threaded: calls on one context must not overlap.
+ *     Work is queued on the context's own HIP stream; functions that return
+ *     host data synchronise that stream.
+ */
+#ifndef FSL_H
+#define FSL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSL_OK            0
+#define FSL_ERR_ARG      -1
+#define FSL_ERR_HIP      -2
+#define FSL_ERR_NOMEM    -3
+#define FSL_ERR_NONFINITE -4   /* CFL reduction gave 0, inf or NaN */
+#define FSL_ERR_IO       -5
+#define FSL_ERR_UNSUPPORTED -6
+
+/* ------------------------------------------------------------------ */
+/* Case input                                                          */
+/* ------------------------------------------------------------------ */
+
+/* The 22 positional items of layer2.ini, in file order
+   (src/initialize.c:18-58; the same order as the 22 float32 of
+   mpi_layer2.bin, src-par/initialize.c:53-77). */
+typedef struct fsl_case {
+    unsigned LEN, HIG, DEP;
+    double deltaX, deltaY, deltaZ, deltaT;
+    unsigned Answer, numstep;
+    double mu_L, Pr_L, Pr_T;
+    double Cs;                 /* holds Cs^2 (layer2.ini:13) */
+    unsigned BL_HIG;
+    double Ua, Va, Wa;
+    int Ns_min, Nst;
+    unsigned f_step;
+    int nStages;
+    double maxCoNum;
+} fsl_case;
+
+/* Replaces the parse of src/initialize.c:18-60: 22 lines read with
+   fgets(buf, 79, f), atoi/atof of each line.  FSL_ERR_IO if a line is missing. */
+int fsl_case_read_ini(const char *path, fsl_case *out);
+/* src-par/initialize.c:26-85: 22 native float32 values, integers converted
+   from float. */
+int fsl_case_read_bin(const char *path, fsl_case *out);
+/* src-par/ini2bin/ini2bin.c:11-81: .ini -> 22 float32 (atof of each line). */
+int fsl_case_ini_to_bin(const char *ini_path, const char *bin_path);
+/* Checks the case for sizes the reference would index out of bounds with
+   (BL_HIG > HIG - HIG/2, src/bounCondOnInterfaces.c:31) and for zeros. */
+int fsl_case_validate(const fsl_case *c, char *msg, size_t msglen);
+
+/* ------------------------------------------------------------------ */
+/* Host helpers                                                        */
+/* ------------------------------------------------------------------ */
+
+/* src/helpers.c:13-17: x <- frac(11 x + pi). */
+double fsl_random(double x);
+/* src/reconstruction.c:17-24 on the host (the device copy is inlined). */
+double fsl_minmod(double x, double y);
+
+/* Disturbance block of src/layer2.c:152-162, advanced once per step. */
+typedef struct fsl_block {
+    double X;          /* LCG state (seed: the number on the reference's stdin) */
+    int counter;       /* steps left before the next draw */
+    unsigned k_min, k_max;
+    double Ud, Vd, Wd;
+} fsl_block;
+void fsl_block_init(fsl_block *b, double seed);
+/* returns 1 when a new block was drawn */
+int fsl_block_advance(fsl_block *b, const fsl_case *c);
+
+/* Initial conditions into a host state (5 cell arrays, ghosts zeroed). */
+/* src/initialize.c:159-187: two streams, 76.4 m/s for j <= HIG/2, 200 above. */
+void fsl_ic_reference(const fsl_case *c, double *U);
+
+/* Spatially developing shear layer of the synthetic benchmark case:
+   U(y) = Um + dU/2 tanh(2 y / delta_w0), y measured from the j = HIG/2 face,
+   uniform p0 and rho0, and seeded uniform perturbations of amplitude
+   pert * dU on u, v, w where |y| < delta_w0 (splitmix64 of seed ^ cell index).
+   E is rebuilt from p0.  u_in (HIG values, may be NULL) receives U(y_j),
+   the matching inflow profile. */
+typedef struct fsl_shear_case {
+    double mach_c;     /* convective Mach number Mc */
+    double ratio;      /* U2 / U1 */
+    double delta_w0;   /* vorticity thickness, m */
+    double reynolds;   /* Re on delta_w0 (sets mu_L = rho0 dU delta_w0 / Re) */
+    double pert;       /* perturbation amplitude as a fraction of dU */
+    double rho0, p0;
+    unsigned long long seed;
+} fsl_shear_case;
+void fsl_shear_defaults(fsl_shear_case *s);
+/* fills U and u_in; returns the mu_L the case implies through Re */
+double fsl_ic_shear_layer(const fsl_case *c, const fsl_shear_case *s, double *U, double *u_in);
+
+/* backup.dat of src/finalize.c:14-34: for i, j: DEP values of each of U1..U5,
+   then deltaT, then step (uint32). */
+int fsl_backup_write(const char *path, const fsl_case *c, const double *U, double deltaT,
+                     unsigned step, int real_bytes /* 8: real = double, 4: float as shipped */);
+/* Reader of src/initialize.c:133-158.  The reference reads step from the
+   bytes right after the fields (where the writer put deltaT); ref_layout = 1
+   reproduces that, 0 reads deltaT then step as written. */
+int fsl_backup_read(const char *path, const fsl_case *c, double *U, double *deltaT,
+                    unsigned *step, int ref_layout, int real_bytes);
+
+/* ------------------------------------------------------------------ */
+/* Device context                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct fsl_ctx fsl_ctx;
+
+#define FSL_SGS_CONSTANT 0   /* src/fluxes.c:195-203, DynamicSmagorinskySGS = 0 */
+#define FSL_SGS_DYNAMIC  1   /* src/turbulence.c:103-507, as shipped */
+#define FSL_SGS_LEARNED  2   /* per-cell MLP for Cs^2, dynamic model as judge */
+
+#define FSL_PREC_FP64    0   /* parity path: FP64, no contraction, reference op order */
+#define FSL_PREC_MIXED   1   /* FP32 stencils, FP64 state / RK / p recovery / CFL */
+
+#define FSL_CFL_CONVECTIVE 0 /* src/helpers.c:23-52: max (rho u / rho) dt/dx, rescale */
+#define FSL_CFL_ACOUSTIC   1 /* dt = cfl / max[(|u|+c)/dx + (|v|+c)/dy + (|w|+c)/dz] */
+
+typedef struct fsl_config {
+    int sgs_model;        /* FSL_SGS_* */
+    int precision;        /* FSL_PREC_* */
+    int cfl_mode;         /* FSL_CFL_* */
+    double cfl;           /* acoustic CFL number (convective mode uses maxCoNum) */
+    int z_quirk;          /* 1: z reconstruction as shipped (src/reconstruction.c:328,340) */
+    int fused;            /* 1: fused stage kernels (3 stages only); 0: one kernel per entry point */
+    double rho_in, p_in;  /* inflow state (reference: 1.0, 1e5) */
+    double p_out;         /* outflow pressure (reference: 1e5) */
+    int device;           /* HIP device ordinal */
+} fsl_config;
+
+/* Reference defaults: dynamic SGS, FP64, convective CFL, z quirk, unfused,
+   inflow 1.0 / 1e5 / 1e5, device 0. */
+void fsl_config_defaults(fsl_config *cfg);
+
+/* Allocates device state for the case (src/initialize.c:96-125 allocates the
+   reference's 51 host arrays).  The inflow profile defaults to the reference
+   step (76.4 / 200 m/s). */
+int fsl_create(const fsl_case *c, const fsl_config *cfg, fsl_ctx **out);
+void fsl_destroy(fsl_ctx *ctx);
+const char *fsl_last_error(const fsl_ctx *ctx);
+/* sizes: cells (LEN+2)(HIG+2)(DEP+2), faces per direction */
+size_t fsl_ncell(const fsl_ctx *ctx);
+size_t fsl_nface(const fsl_ctx *ctx, int dir);
+/* bytes of device memory the context holds */
+size_t fsl_device_bytes(const fsl_ctx *ctx);
+
+/* inflow velocity per cell row j = 1..HIG (HIG values) */
+int fsl_set_inflow(fsl_ctx *ctx, const double *u_in);
+
+/* State transfer.  which: 0 = U (step state), 1 = Up (stage state).
+   Layout: 5 dense cell arrays.  Upload after Initialize / restart; download
+   before Output / Probes / Finalize. */
+int fsl_upload_state(fsl_ctx *ctx, int which, const double *U);
+int fsl_download_state(fsl_ctx *ctx, int which, double *U);
+/* Face arrays of the unfused path: id 0..4 xU*, 5..9 U*x, 10..14 yU*,
+   15..19 U*y, 20..24 zU*, 25..29 U*z (after Fluxes, the xU*, yU*, zU* arrays hold the
+   fluxes, as src/def.h:6-22 aliases them). */
+int fsl_download_face(fsl_ctx *ctx, int id, double *out);
+int fsl_upload_face(fsl_ctx *ctx, int id, const double *in);
+/* mu_SGS cell array (ghost layer 0) */
+int fsl_download_mu_sgs(fsl_ctx *ctx, double *out);
+int fsl_upload_mu_sgs(fsl_ctx *ctx, const double *in);
+
+/* Scalars.  fsl_set_dt sets deltaT and deltaT_X/Y/Z = deltaT / delta. */
+int fsl_set_dt(fsl_ctx *ctx, double deltaT);
+double fsl_get_dt(const fsl_ctx *ctx);
+int fsl_set_block(fsl_ctx *ctx, unsigned k_min, unsigned k_max, double Ud, double Vd, double Wd);
+/* stage-input selector of the reference (U_ = U or U_ = Up) */
+int fsl_get_cur(const fsl_ctx *ctx);
+int fsl_set_cur(fsl_ctx *ctx, int is_p);
+
+/* --- per-entry-point operations (unfused path) --- */
+/* src/boundCondInGhostCells.c:10-96 on the stage input U_ */
+int fsl_ghost_fill(fsl_ctx *ctx);
+/* src/reconstruction.c:26-345: 30 face arrays from U_ */
+int fsl_reconstruct(fsl_ctx *ctx);
+/* src/bounCondOnInterfaces.c:11-110 */
+int fsl_iface_bc(fsl_ctx *ctx);
+/* src/fluxes.c:12-573 (calls the dynamic model first when configured) */
+int fsl_fluxes(fsl_ctx *ctx);
+/* src/evolution.c:15-27 (2 stages falls through into the 3-stage update) */
+int fsl_evolution(fsl_ctx *ctx, int numStages, int stage);
+/* One scheme's update only: scheme 3 = src/evolution.c:29-161,
+   scheme 2 = src/evolution.c:164-245 */
+int fsl_rk_update(fsl_ctx *ctx, int scheme, int stage);
+/* src/helpers.c:23-52: device max-reduction, then deltaT *= maxCoNum / maxCo
+   (convective) or deltaT = cfl / max(...) (acoustic).  maxCo may be NULL. */
+int fsl_check_co_num(fsl_ctx *ctx, double *maxCo);
+/* src/turbulence.c:103-507 on the step state U -> mu_SGS */
+int fsl_dynamic_smagorinsky(fsl_ctx *ctx);
+/* src/turbulence.c:17-100 with kernel h (the model uses {1/3,1/3,1/3}):
+   host cell array in -> host cell array out (the returned array of the
+   reference) */
+int fsl_filter(fsl_ctx *ctx, const double *in, double *out, const double h[3]);
+
+/* --- fused path --- */
+/* One RK stage (ghost fill, reconstruction, interface BC, fluxes, update)
+   with fused kernels; stage in 1..3. */
+int fsl_stage(fsl_ctx *ctx, int stage);
+
+/* --- driver (ANSI C over the operations above) --- */
+/* One time step in the order of src/layer2.c:141-185: CFL rescale, block
+   draw, nStages stages.  The block state lives in the context. */
+int fsl_step(fsl_ctx *ctx);
+int fsl_run(fsl_ctx *ctx, int nsteps);
+int fsl_set_lcg(fsl_ctx *ctx, double X, int counter);
+int fsl_get_lcg(const fsl_ctx *ctx, double *X, int *counter, unsigned *k_min, unsigned *k_max,
+                double *Ud, double *Vd, double *Wd);
+unsigned fsl_get_step(const fsl_ctx *ctx);
+int fsl_set_step(fsl_ctx *ctx, unsigned step);
+int fsl_get_block_state(const fsl_ctx *ctx, fsl_block *b);
+int fsl_set_block_state(fsl_ctx *ctx, const fsl_block *b);
+int fsl_get_case(const fsl_ctx *ctx, fsl_case *out);
+int fsl_get_config(const fsl_ctx *ctx, fsl_config *out);
+int fsl_synchronize(fsl_ctx *ctx);
+
+/* --- diagnostics (device FP64 reductions) --- */
+/* out[0..4]: sums of rho, rho u, rho v, rho w, E over the interior */
+int fsl_conservation_sums(fsl_ctx *ctx, double out[5]);
+
+/* --- measurement --- */
+/* When on, every kernel launch is bracketed by HIP events on the context's
+   stream; fsl_profile_read returns, per kernel id, the accumulated milliseconds
+   and launch count.  Names by fsl_kernel_name(id). */
+int fsl_profile_enable(fsl_ctx *ctx, int on);
+int fsl_profile_reset(fsl_ctx *ctx);
+int fsl_profile_read(fsl_ctx *ctx, int nmax, double *ms, long long *count);
+const char *fsl_kernel_name(int id);
+int fsl_kernel_count(void);
+/* device copy of n doubles (calibration of HBM counters against a plain copy) */
+int fsl_copy_kernel(fsl_ctx *ctx, size_t n, int reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
