@@ -778,6 +778,13 @@ CtxView ctx_view(fsl_ctx* c)
 }
 
 void ctx_prof(fsl_ctx* c, int id, int end) { prof_mark(c, id, end); }
+int ctx_ghost(fsl_ctx* c, const double* state)
+{
+    prof_mark(c, K_GHOST, 0);
+    launch_ghost_fill(c->stream, st5(const_cast<double*>(state), c->g.G), c->g, bnd_now(c));
+    prof_mark(c, K_GHOST, 1);
+    return check_launch(c, "ghost_fill");
+}
 int ctx_check(fsl_ctx* c, const char* what) { return check_launch(c, what); }
 int ctx_sgs(fsl_ctx* c) { return fsl_dynamic_smagorinsky(c); }
 void ctx_set_cur(fsl_ctx* c, int is_p) { c->cur_is_p = is_p; }

@@ -1,12 +1,399 @@
-// fsl_fused.hip -- fused stage kernels (placeholder until implemented).
+// fsl_fused.hip -- fused stage kernel: reconstruction, interface boundary
+// states, Riemann solver, viscous/SGS fluxes and the TVD-RK update of one
+// stage in a single pass over the stage state.
+//
+// Reference functions fused here (nikola-m/freeShearLayer):
+//   src/reconstruction.c:26-345, src/bounCondOnInterfaces.c:11-110,
+//   src/fluxes.c:57-571, src/evolution.c:29-161 (three-stage scheme).
+// The arithmetic is the device building blocks of fsl_device.hpp in the
+// reference's operation order, so the result is bit-identical to the unfused
+// path (and to src/ with real = double).
+//
+// Decomposition.  A workgroup owns a 14 x 14 tile of (j, k) columns plus a
+// one-cell halo ring (16 x 16 threads, k on the fast lane index so every row
+// access is a coalesced 128-byte run) and marches along x over a segment of
+// cells.  Per plane i each thread computes, once, its cell's primitives and
+// its x, y and z face states; the x faces never leave registers (the march
+// carries the previous plane's + state and flux), the y and z face states,
+// the fluxes and the per-cell viscous data are exchanged through LDS, and a
+// three-plane LDS ring of 1/rho-velocities feeds the tangential gradients.
+// HBM traffic per cell and stage is the stage state (plus halo re-reads that
+// hit L2), the step state for stages 2 and 3, mu_SGS and the new state.
 #include "fsl.h"
 #include "fsl_internal.hpp"
 
 namespace fsl {
+
+struct CtxView {
+    hipStream_t stream;
+    Geo g;
+    Phys ph;
+    Bnd bnd;
+    double *U, *Up, *Uq, *mu;
+    double dtX, dtY, dtZ;
+    int cur_is_p;
+    int sgs_model;
+    int precision;
+};
+CtxView ctx_view(fsl_ctx* c);
+void ctx_prof(fsl_ctx* c, int id, int end);
+int ctx_check(fsl_ctx* c, const char* what);
+int ctx_sgs(fsl_ctx* c);
+int ctx_ghost(fsl_ctx* c, const double* state);
+void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
+
+namespace {
+
+constexpr int BJ = 16, BK = 16;
+constexpr int TJ = BJ - 2, TK = BK - 2;
+constexpr int NT = BJ * BK;
+
+struct FusedArgs {
+    CState5 in;      // stage input, ghost layer filled
+    CState5 base;    // step state (stages 2 and 3)
+    State5 out;      // new state (interior written)
+    const double* mu;
+    Geo g;
+    Phys ph;
+    Bnd b;
+    double dtX, dtY, dtZ;
+    int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+};
+
+__device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
+{
+#pragma unroll
+    for (int f = 0; f < 5; f++) q[f] = s.f[f][c];
+}
+
+__device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
+
+template <int STAGE>
+__global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
+{
+    __shared__ double s_vr[3][3][NT];   // 1/rho velocities, planes i-1, i, i+1
+    __shared__ double s_visc[5][NT];    // rho u / rho (3), T, rho of plane i
+    __shared__ double s_ym[5][NT];      // - face states along y
+    __shared__ double s_zm[5][NT];      // - face states along z
+    __shared__ double s_fy[5][NT];      // y fluxes through the + face
+    __shared__ double s_fz[5][NT];      // z fluxes through the + face
+
+    const Geo& g = a.g;
+    const int L = g.L, H = g.H, D = g.D;
+
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        // consecutive logical tiles on one XCD (workgroups are dealt to the
+        // eight XCDs round-robin), so neighbours share that XCD's L2
+        const int per = gridDim.x >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    if (bid >= a.nblocks) return;
+    const int tk = bid % a.ntk;
+    const int rem = bid / a.ntk;
+    const int tj = rem % a.ntj;
+    const int seg = rem / a.ntj;
+    const int k0 = 1 + tk * TK, j0 = 1 + tj * TJ;
+    const int i0 = 1 + seg * a.segL;
+    const int i1 = min(i0 + a.segL, L + 1);
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int c = ty * BK + tx;
+    const int j = j0 - 1 + ty, k = k0 - 1 + tx;
+    const int jc = min(j, H + 1), kc = min(k, D + 1);
+    const bool interior = tx >= 1 && tx <= TK && ty >= 1 && ty <= TJ && j <= H && k <= D;
+    const bool yface = tx >= 1 && tx <= TK && ty <= TJ && k <= D && j <= H;
+    const bool zface = ty >= 1 && ty <= TJ && tx <= TK && j <= H && k <= D;
+    // y-recon neighbours (clamped: ghost rows never use their y states)
+    const int jm = max(jc - 1, 0), jp = min(jc + 1, H + 1);
+    // z-recon neighbours; the ghost columns reconstruct their periodic partner
+    // (k = 0 -> cell D's + state, k = D+1 -> cell 1's - state)
+    int zkm = kc - 1, zkp = kc + 1;
+    if (kc == 0) { zkm = D - 1; zkp = D + 1; }
+    else if (kc == D + 1) { zkm = 0; zkp = 2; }
+    const long long sj = g.sj, si = g.si;
+    const long long col = (long long)jc * sj + kc;   // (j, k) offset of this column
+
+    double w0[5], w1[5], w2[5];   // stage state at planes i-1, i, i+1 (own column)
+    double xp_prev[5];            // + x state of plane i-1
+    double vd_prev[3], T_prev = 0., R_prev = 0.;
+    double Fx_prev[5];
+    double ly[5], lz[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) { xp_prev[f] = 0.; Fx_prev[f] = 0.; ly[f] = 0.; lz[f] = 0.; }
+    vd_prev[0] = vd_prev[1] = vd_prev[2] = 0.;
+
+    {
+        const int ia = max(i0 - 2, 0), ib = i0 - 1;
+        load5(a.in, (long long)ia * si + col, w1);
+        load5(a.in, (long long)ib * si + col, w2);
+        double vr[3];
+        vel_recip(w2[0], w2[1], w2[2], w2[3], vr);
+        const int r = ring(ib);
+#pragma unroll
+        for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr[e];
+    }
+
+    for (int i = i0 - 1; i <= i1; i++) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
+        load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
+        __syncthreads();   // the previous plane's LDS readers are done
+        {
+            double vr[3];
+            vel_recip(w2[0], w2[1], w2[2], w2[3], vr);
+            const int r = ring(i + 1);
+#pragma unroll
+            for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr[e];
+        }
+        Prim<double> P;
+        cell_prims(w1, P);
+        double xp[5], xm[5];
+        recon_dir<1, 2, 3>(P, w2, w0, false, xp, xm);
+        double vd[3], Tc;
+        vel_div_T(w1[0], w1[1], w1[2], w1[3], w1[4], vd, Tc);
+        const double Rc = w1[0];
+        s_visc[0][c] = vd[0];
+        s_visc[1][c] = vd[1];
+        s_visc[2][c] = vd[2];
+        s_visc[3][c] = Tc;
+        s_visc[4][c] = Rc;
+        const bool inseg = (i >= i0 && i < i1);
+        double yp[5], zp[5];
+        if (inseg) {
+            const long long pl = (long long)i * si;
+            double qp[5], qm[5], tmp[5];
+            load5(a.in, pl + (long long)jp * sj + kc, qp);
+            load5(a.in, pl + (long long)jm * sj + kc, qm);
+            recon_dir<2, 3, 1>(P, qp, qm, false, yp, tmp);
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_ym[f][c] = tmp[f];
+            load5(a.in, pl + (long long)jc * sj + zkp, qp);
+            load5(a.in, pl + (long long)jc * sj + zkm, qm);
+            recon_dir<3, 1, 2>(P, qp, qm, a.ph.z_quirk != 0, zp, tmp);
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_zm[f][c] = tmp[f];
+        }
+        __syncthreads();
+
+        // ---- x face i - 1/2 and the update of cell i - 1 ----
+        if (i >= i0 && interior) {
+            double l[5], r[5];
+            if (i == 1) {
+                inflow_face(a.b, H, j - 1, k - 1, l);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; f++) l[f] = xp_prev[f];
+            }
+            if (i == L + 1) {
+#pragma unroll
+                for (int f = 0; f < 5; f++) r[f] = xp_prev[f];
+                outflow_state(a.b.p_out, r);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; f++) r[f] = xm[f];
+            }
+            double R, vel[3], Pf;
+            riemann<0>(l, r, R, vel, Pf);
+            const int rc = ring(i), rb = ring(i - 1);
+            double gr[3][3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                gr[q][0] = (vd[q] - vd_prev[q]) * a.ph.rdel[0];
+                gr[q][1] = (s_vr[rc][q][c + BK] + s_vr[rb][q][c + BK] - s_vr[rc][q][c - BK] -
+                            s_vr[rb][q][c - BK]) * a.ph.r4del[1];
+                gr[q][2] = (s_vr[rc][q][c + 1] + s_vr[rb][q][c + 1] - s_vr[rc][q][c - 1] -
+                            s_vr[rb][q][c - 1]) * a.ph.r4del[2];
+            }
+            double mu_dyn = 0.;
+            if (a.ph.dynamic) {
+                const long long m = (long long)i * si + col;
+                mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
+            }
+            double Fx[5];
+            face_flux<0>(a.ph, R, vel, Pf, vd, vd_prev, Tc, T_prev, Rc, R_prev, gr, mu_dyn, Fx);
+            if (i - 1 >= i0) {
+                const long long cc = (long long)(i - 1) * si + col;
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    const double lx = a.dtX * (Fx_prev[f] - Fx[f]);
+                    const double s = w0[f] + lx + ly[f] + lz[f];
+                    double o;
+                    if (STAGE == 1) o = s;
+                    else if (STAGE == 2) o = 0.75 * a.base.f[f][cc] + 0.25 * s;
+                    else o = (1. / 3.) * a.base.f[f][cc] + (2. / 3.) * s;
+                    a.out.f[f][cc] = o;
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
+        }
+
+        // ---- y and z faces of plane i ----
+        if (inseg) {
+            const int rp = ring(i + 1), rc = ring(i), rm = ring(i - 1);
+            if (yface) {
+                double l[5], r[5];
+                if (j == 0) {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) l[f] = s_ym[f][c + BK];
+                    l[2] = -l[2];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) l[f] = yp[f];
+                }
+                if (j == H) {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) r[f] = yp[f];
+                    r[2] = -r[2];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) r[f] = s_ym[f][c + BK];
+                }
+                double R, vel[3], Pf;
+                riemann<1>(l, r, R, vel, Pf);
+                double vf[3] = {s_visc[0][c + BK], s_visc[1][c + BK], s_visc[2][c + BK]};
+                const double Tf = s_visc[3][c + BK], Rf = s_visc[4][c + BK];
+                double gr[3][3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    gr[q][1] = (vf[q] - vd[q]) * a.ph.rdel[1];
+                    gr[q][0] = (s_vr[rp][q][c + BK] + s_vr[rp][q][c] - s_vr[rm][q][c + BK] -
+                                s_vr[rm][q][c]) * a.ph.r4del[0];
+                    gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
+                                s_vr[rc][q][c - 1]) * a.ph.r4del[2];
+                }
+                double mu_dyn = 0.;
+                if (a.ph.dynamic) {
+                    const long long m = (long long)i * si + (long long)(jc + 1) * sj + kc;
+                    mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
+                }
+                double Fy[5];
+                face_flux<1>(a.ph, R, vel, Pf, vf, vd, Tf, Tc, Rf, Rc, gr, mu_dyn, Fy);
+#pragma unroll
+                for (int f = 0; f < 5; f++) s_fy[f][c] = Fy[f];
+            }
+            if (zface) {
+                double l[5], r[5];
+#pragma unroll
+                for (int f = 0; f < 5; f++) { l[f] = zp[f]; r[f] = s_zm[f][c + 1]; }
+                double R, vel[3], Pf;
+                riemann<2>(l, r, R, vel, Pf);
+                double vf[3] = {s_visc[0][c + 1], s_visc[1][c + 1], s_visc[2][c + 1]};
+                const double Tf = s_visc[3][c + 1], Rf = s_visc[4][c + 1];
+                double gr[3][3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    gr[q][2] = (vf[q] - vd[q]) * a.ph.rdel[2];
+                    gr[q][0] = (s_vr[rp][q][c + 1] + s_vr[rp][q][c] - s_vr[rm][q][c + 1] -
+                                s_vr[rm][q][c]) * a.ph.r4del[0];
+                    gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
+                                s_vr[rc][q][c - BK]) * a.ph.r4del[1];
+                }
+                double mu_dyn = 0.;
+                if (a.ph.dynamic) {
+                    const long long m = (long long)i * si + (long long)jc * sj + kc + 1;
+                    mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
+                }
+                double Fz[5];
+                face_flux<2>(a.ph, R, vel, Pf, vf, vd, Tf, Tc, Rf, Rc, gr, mu_dyn, Fz);
+#pragma unroll
+                for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
+            }
+        }
+        __syncthreads();
+        if (inseg && interior) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                ly[f] = a.dtY * (s_fy[f][c - BK] - s_fy[f][c]);
+                lz[f] = a.dtZ * (s_fz[f][c - 1] - s_fz[f][c]);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
+        vd_prev[0] = vd[0];
+        vd_prev[1] = vd[1];
+        vd_prev[2] = vd[2];
+        T_prev = Tc;
+        R_prev = Rc;
+    }
+}
+
+struct Launch {
+    dim3 grid;
+    int segL, nseg, ntj, ntk, nblocks, remap;
+};
+
+Launch plan(const Geo& g)
+{
+    Launch p;
+    p.ntj = (g.H + TJ - 1) / TJ;
+    p.ntk = (g.D + TK - 1) / TK;
+    const int cols = p.ntj * p.ntk;
+    // about four resident waves of workgroups over 256 CUs at 2 per CU
+    const int target = 2048;
+    int nseg = (target + cols - 1) / cols;
+    if (nseg > g.L) nseg = g.L;
+    if (nseg < 1) nseg = 1;
+    int segL = (g.L + nseg - 1) / nseg;
+    if (segL < 4 && g.L >= 4) segL = 4;
+    nseg = (g.L + segL - 1) / segL;
+    p.segL = segL;
+    p.nseg = nseg;
+    p.nblocks = nseg * cols;
+    p.remap = 1;
+    const int padded = (p.nblocks + 7) / 8 * 8;
+    p.grid = dim3(padded);
+    return p;
+}
+
+}  // namespace
+
 int fused_stage(fsl_ctx* ctx, int stage)
 {
-    (void)stage;
-    return ctx_err(ctx, FSL_ERR_UNSUPPORTED, "fused path not built");
+    CtxView v = ctx_view(ctx);
+    const double* in = stage == 1 ? v.U : (stage == 2 ? v.Up : v.Uq);
+    double* out = stage == 1 ? v.Up : (stage == 2 ? v.Uq : v.U);
+    if (!v.Uq) return ctx_err(ctx, FSL_ERR_ARG, "fused path needs the third stage buffer (create with fused = 1)");
+    int rc = ctx_ghost(ctx, in);
+    if (rc) return rc;
+    if (stage == 1 && v.sgs_model == FSL_SGS_DYNAMIC) {
+        // the model reads the step state only: once per step (src/fluxes.c:53)
+        rc = ctx_sgs(ctx);
+        if (rc) return rc;
+    }
+    FusedArgs a;
+    for (int f = 0; f < 5; f++) {
+        a.in.f[f] = in + f * v.g.G;
+        a.base.f[f] = v.U + f * v.g.G;
+        a.out.f[f] = out + f * v.g.G;
+    }
+    a.mu = v.mu;
+    a.g = v.g;
+    a.ph = v.ph;
+    a.b = v.bnd;
+    a.dtX = v.dtX;
+    a.dtY = v.dtY;
+    a.dtZ = v.dtZ;
+    const Launch p = plan(v.g);
+    a.segL = p.segL;
+    a.nseg = p.nseg;
+    a.ntj = p.ntj;
+    a.ntk = p.ntk;
+    a.nblocks = p.nblocks;
+    a.xcd_remap = p.remap;
+    const dim3 block(BK, BJ);
+    ctx_prof(ctx, K_FUSED_XYZ, 0);
+    if (stage == 1) hipLaunchKernelGGL(k_fused<1>, p.grid, block, 0, v.stream, a);
+    else if (stage == 2) hipLaunchKernelGGL(k_fused<2>, p.grid, block, 0, v.stream, a);
+    else hipLaunchKernelGGL(k_fused<3>, p.grid, block, 0, v.stream, a);
+    ctx_prof(ctx, K_FUSED_XYZ, 1);
+    rc = ctx_check(ctx, "fused stage");
+    if (rc) return rc;
+    // stage inputs: U, then Up, then Uq; the state after stage 3 is U
+    ctx_set_cur(ctx, stage == 3 ? 0 : 1);
+    return FSL_OK;
 }
+
 }  // namespace fsl

@@ -1,0 +1,57 @@
+"""GPU parity of the fused stage path: one kernel per stage (plus the ghost
+fill and, once per step, the dynamic model) against the CPU oracle, bit for
+bit, on meshes that do and do not divide the 14 x 14 workgroup tile, and
+against the unfused device path at the benchmark size."""
+import numpy as np
+import pytest
+
+import freeshearlayer_amd as fsl
+from fsl_test_util import assert_bitwise, make_pair, random_state
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [(16, 12, 8), (5, 10, 3), (30, 29, 15), (9, 44, 40), (40, 10, 1)]
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+@pytest.mark.parametrize("dynamic", [0, 1])
+def test_fused_steps_match_oracle(mesh, dynamic):
+    orc, ctx = make_pair(*mesh, dynamic=dynamic, seed=31, fused=1)
+    for n in range(3):
+        ctx.step()
+        orc.step()
+        assert ctx.dt == orc.get("dt"), "dt at step %d" % (n + 1)
+        assert_bitwise(orc.state()[:, 1:-1, 1:-1, 1:-1], ctx.download_state()[:, 1:-1, 1:-1, 1:-1],
+                       "interior U after step %d" % (n + 1))
+
+
+@pytest.mark.parametrize("dynamic", [0, 1])
+def test_fused_reference_case(dynamic):
+    orc, ctx = make_pair(16, 12, 8, dynamic=dynamic, fused=1)
+    for n in range(8):
+        ctx.step()
+        orc.step()
+        assert_bitwise(orc.state()[:, 1:-1, 1:-1, 1:-1], ctx.download_state()[:, 1:-1, 1:-1, 1:-1],
+                       "interior U after step %d" % (n + 1))
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_fused_equals_unfused_at_bench_size(quirk):
+    """Size-independent property at 192 x 128 x 96: the fused and unfused
+    device paths give the same bits after two steps."""
+    L, H, D = 192, 128, 96
+    case = fsl.case_reference(L, H, D)
+    U, u_in, mu = fsl.ic_shear_layer(case)
+    case.mu_L = mu
+    out = []
+    for fused in (0, 1):
+        cfg = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC, fused=fused, z_quirk=quirk,
+                         cfl_mode=fsl.FSL_CFL_ACOUSTIC, cfl=0.5)
+        with fsl.Context(case, cfg) as ctx:
+            ctx.set_inflow(u_in)
+            ctx.upload_state(U)
+            ctx.set_lcg(0.317391239817, 0)
+            ctx.run(2)
+            out.append(ctx.download_state()[:, 1:-1, 1:-1, 1:-1])
+    assert_bitwise(out[0], out[1], "fused vs unfused")
+    assert np.all(np.isfinite(out[1]))
