@@ -160,11 +160,17 @@ def main(argv=None):
         name, (ms, cnt) = dom
         avg_s = ms / cnt / 1e3
         b = algorithmic_bytes(name, L, H, D, a.sgs == "dynamic")
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+        wl = "%s %s-SGS %s path, %dx%dx%d cells, 3-stage TVD RK" % (a.precision.upper(), a.sgs, a.path,
+                                                                     L, H, D)
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get("%s@%s" % (name, wl))
         if b is not None:
             ach = b / avg_s / 1e9
             roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                        "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
                         "alg_bytes_per_launch": int(b)}
         kernels = {k: {"ms_total": round(v[0], 3), "launches": int(v[1]),
                        "avg_us": round(v[0] / v[1] * 1e3, 2)} for k, v in sorted(prof.items())}
