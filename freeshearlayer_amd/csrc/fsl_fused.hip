@@ -19,6 +19,8 @@
 // three-plane LDS ring of 1/rho-velocities feeds the tangential gradients.
 // HBM traffic per cell and stage is the stage state (plus halo re-reads that
 // hit L2), the step state for stages 2 and 3, mu_SGS and the new state.
+#include <stdlib.h>
+
 #include "fsl.h"
 #include "fsl_internal.hpp"
 
@@ -45,10 +47,6 @@ int ctx_err(fsl_ctx* c, int code, const char* msg);
 
 namespace {
 
-constexpr int BJ = 16, BK = 16;
-constexpr int TJ = BJ - 2, TK = BK - 2;
-constexpr int NT = BJ * BK;
-
 struct FusedArgs {
     CState5 in;      // stage input, ghost layer filled
     CState5 base;    // step state (stages 2 and 3)
@@ -69,15 +67,18 @@ __device__ __forceinline__ void load5(const CState5& s, long long c, double q[5]
 
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 
-template <int STAGE>
-__global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
+// BJ x BK threads (BK on the fast lane index), MINW: minimum waves per SIMD
+// the register allocation must allow.
+template <int STAGE, int BJ, int BK, int MINW>
+__global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
 {
+    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
     __shared__ double s_vr[3][3][NT];   // 1/rho velocities, planes i-1, i, i+1
-    __shared__ double s_visc[5][NT];    // rho u / rho (3), T, rho of plane i
+    __shared__ double s_visc[2][5][NT]; // rho u / rho (3), T, rho of planes i-1, i
     __shared__ double s_ym[5][NT];      // - face states along y
     __shared__ double s_zm[5][NT];      // - face states along z
-    __shared__ double s_fy[5][NT];      // y fluxes through the + face
-    __shared__ double s_fz[5][NT];      // z fluxes through the + face
+    __shared__ double s_fy[5][NT];      // own + y state, then the y flux through it
+    __shared__ double s_fz[5][NT];      // own + z state, then the z flux through it
 
     const Geo& g = a.g;
     const int L = g.L, H = g.H, D = g.D;
@@ -117,12 +118,10 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
 
     double w0[5], w1[5], w2[5];   // stage state at planes i-1, i, i+1 (own column)
     double xp_prev[5];            // + x state of plane i-1
-    double vd_prev[3], T_prev = 0., R_prev = 0.;
     double Fx_prev[5];
     double ly[5], lz[5];
 #pragma unroll
     for (int f = 0; f < 5; f++) { xp_prev[f] = 0.; Fx_prev[f] = 0.; ly[f] = 0.; lz[f] = 0.; }
-    vd_prev[0] = vd_prev[1] = vd_prev[2] = 0.;
 
     {
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
@@ -140,41 +139,40 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
         load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
         __syncthreads();   // the previous plane's LDS readers are done
+        const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {
             double vr[3];
             vel_recip(w2[0], w2[1], w2[2], w2[3], vr);
             const int r = ring(i + 1);
 #pragma unroll
             for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr[e];
+            double vd[3], Tc;
+            vel_div_T(w1[0], w1[1], w1[2], w1[3], w1[4], vd, Tc);
+            s_visc[vc][0][c] = vd[0];
+            s_visc[vc][1][c] = vd[1];
+            s_visc[vc][2][c] = vd[2];
+            s_visc[vc][3][c] = Tc;
+            s_visc[vc][4][c] = w1[0];
         }
         Prim<double> P;
         cell_prims(w1, P);
-        double xp[5], xm[5];
-        recon_dir<1, 2, 3>(P, w2, w0, false, xp, xm);
-        double vd[3], Tc;
-        vel_div_T(w1[0], w1[1], w1[2], w1[3], w1[4], vd, Tc);
-        const double Rc = w1[0];
-        s_visc[0][c] = vd[0];
-        s_visc[1][c] = vd[1];
-        s_visc[2][c] = vd[2];
-        s_visc[3][c] = Tc;
-        s_visc[4][c] = Rc;
         const bool inseg = (i >= i0 && i < i1);
-        double yp[5], zp[5];
         if (inseg) {
             const long long pl = (long long)i * si;
-            double qp[5], qm[5], tmp[5];
+            double qp[5], qm[5], pl5[5], mi5[5];
             load5(a.in, pl + (long long)jp * sj + kc, qp);
             load5(a.in, pl + (long long)jm * sj + kc, qm);
-            recon_dir<2, 3, 1>(P, qp, qm, false, yp, tmp);
+            recon_dir<2, 3, 1>(P, qp, qm, false, pl5, mi5);
 #pragma unroll
-            for (int f = 0; f < 5; f++) s_ym[f][c] = tmp[f];
+            for (int f = 0; f < 5; f++) { s_fy[f][c] = pl5[f]; s_ym[f][c] = mi5[f]; }
             load5(a.in, pl + (long long)jc * sj + zkp, qp);
             load5(a.in, pl + (long long)jc * sj + zkm, qm);
-            recon_dir<3, 1, 2>(P, qp, qm, a.ph.z_quirk != 0, zp, tmp);
+            recon_dir<3, 1, 2>(P, qp, qm, a.ph.z_quirk != 0, pl5, mi5);
 #pragma unroll
-            for (int f = 0; f < 5; f++) s_zm[f][c] = tmp[f];
+            for (int f = 0; f < 5; f++) { s_fz[f][c] = pl5[f]; s_zm[f][c] = mi5[f]; }
         }
+        double xp[5], xm[5];
+        recon_dir<1, 2, 3>(P, w2, w0, false, xp, xm);
         __syncthreads();
 
         // ---- x face i - 1/2 and the update of cell i - 1 ----
@@ -197,10 +195,12 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
             double R, vel[3], Pf;
             riemann<0>(l, r, R, vel, Pf);
             const int rc = ring(i), rb = ring(i - 1);
+            const double vf[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
+            const double vbk[3] = {s_visc[vb][0][c], s_visc[vb][1][c], s_visc[vb][2][c]};
             double gr[3][3];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                gr[q][0] = (vd[q] - vd_prev[q]) * a.ph.rdel[0];
+                gr[q][0] = (vf[q] - vbk[q]) * a.ph.rdel[0];
                 gr[q][1] = (s_vr[rc][q][c + BK] + s_vr[rb][q][c + BK] - s_vr[rc][q][c - BK] -
                             s_vr[rb][q][c - BK]) * a.ph.r4del[1];
                 gr[q][2] = (s_vr[rc][q][c + 1] + s_vr[rb][q][c + 1] - s_vr[rc][q][c - 1] -
@@ -212,7 +212,8 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                 mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
             }
             double Fx[5];
-            face_flux<0>(a.ph, R, vel, Pf, vd, vd_prev, Tc, T_prev, Rc, R_prev, gr, mu_dyn, Fx);
+            face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
+                         s_visc[vb][4][c], gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
                 const long long cc = (long long)(i - 1) * si + col;
 #pragma unroll
@@ -229,10 +230,14 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
 #pragma unroll
             for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
         }
+#pragma unroll
+        for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
 
         // ---- y and z faces of plane i ----
         if (inseg) {
             const int rp = ring(i + 1), rc = ring(i), rm = ring(i - 1);
+            const double vbk[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
+            const double Tb = s_visc[vc][3][c], Rb = s_visc[vc][4][c];
             if (yface) {
                 double l[5], r[5];
                 if (j == 0) {
@@ -241,11 +246,11 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                     l[2] = -l[2];
                 } else {
 #pragma unroll
-                    for (int f = 0; f < 5; f++) l[f] = yp[f];
+                    for (int f = 0; f < 5; f++) l[f] = s_fy[f][c];
                 }
                 if (j == H) {
 #pragma unroll
-                    for (int f = 0; f < 5; f++) r[f] = yp[f];
+                    for (int f = 0; f < 5; f++) r[f] = s_fy[f][c];
                     r[2] = -r[2];
                 } else {
 #pragma unroll
@@ -253,12 +258,11 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                 }
                 double R, vel[3], Pf;
                 riemann<1>(l, r, R, vel, Pf);
-                double vf[3] = {s_visc[0][c + BK], s_visc[1][c + BK], s_visc[2][c + BK]};
-                const double Tf = s_visc[3][c + BK], Rf = s_visc[4][c + BK];
+                const double vf[3] = {s_visc[vc][0][c + BK], s_visc[vc][1][c + BK], s_visc[vc][2][c + BK]};
                 double gr[3][3];
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
-                    gr[q][1] = (vf[q] - vd[q]) * a.ph.rdel[1];
+                    gr[q][1] = (vf[q] - vbk[q]) * a.ph.rdel[1];
                     gr[q][0] = (s_vr[rp][q][c + BK] + s_vr[rp][q][c] - s_vr[rm][q][c + BK] -
                                 s_vr[rm][q][c]) * a.ph.r4del[0];
                     gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
@@ -270,22 +274,22 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                     mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
                 }
                 double Fy[5];
-                face_flux<1>(a.ph, R, vel, Pf, vf, vd, Tf, Tc, Rf, Rc, gr, mu_dyn, Fy);
+                face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + BK], Tb, s_visc[vc][4][c + BK], Rb,
+                             gr, mu_dyn, Fy);
 #pragma unroll
                 for (int f = 0; f < 5; f++) s_fy[f][c] = Fy[f];
             }
             if (zface) {
                 double l[5], r[5];
 #pragma unroll
-                for (int f = 0; f < 5; f++) { l[f] = zp[f]; r[f] = s_zm[f][c + 1]; }
+                for (int f = 0; f < 5; f++) { l[f] = s_fz[f][c]; r[f] = s_zm[f][c + 1]; }
                 double R, vel[3], Pf;
                 riemann<2>(l, r, R, vel, Pf);
-                double vf[3] = {s_visc[0][c + 1], s_visc[1][c + 1], s_visc[2][c + 1]};
-                const double Tf = s_visc[3][c + 1], Rf = s_visc[4][c + 1];
+                const double vf[3] = {s_visc[vc][0][c + 1], s_visc[vc][1][c + 1], s_visc[vc][2][c + 1]};
                 double gr[3][3];
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
-                    gr[q][2] = (vf[q] - vd[q]) * a.ph.rdel[2];
+                    gr[q][2] = (vf[q] - vbk[q]) * a.ph.rdel[2];
                     gr[q][0] = (s_vr[rp][q][c + 1] + s_vr[rp][q][c] - s_vr[rm][q][c + 1] -
                                 s_vr[rm][q][c]) * a.ph.r4del[0];
                     gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
@@ -297,7 +301,8 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                     mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
                 }
                 double Fz[5];
-                face_flux<2>(a.ph, R, vel, Pf, vf, vd, Tf, Tc, Rf, Rc, gr, mu_dyn, Fz);
+                face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + 1], Tb, s_visc[vc][4][c + 1], Rb,
+                             gr, mu_dyn, Fz);
 #pragma unroll
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
@@ -310,29 +315,23 @@ __global__ void __launch_bounds__(NT, 2) k_fused(FusedArgs a)
                 lz[f] = a.dtZ * (s_fz[f][c - 1] - s_fz[f][c]);
             }
         }
-#pragma unroll
-        for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
-        vd_prev[0] = vd[0];
-        vd_prev[1] = vd[1];
-        vd_prev[2] = vd[2];
-        T_prev = Tc;
-        R_prev = Rc;
     }
 }
 
 struct Launch {
-    dim3 grid;
+    dim3 grid, block;
     int segL, nseg, ntj, ntk, nblocks, remap;
 };
 
-Launch plan(const Geo& g)
+Launch plan(const Geo& g, int BJ, int BK, int per_cu)
 {
+    const int TJ = BJ - 2, TK = BK - 2;
     Launch p;
     p.ntj = (g.H + TJ - 1) / TJ;
     p.ntk = (g.D + TK - 1) / TK;
     const int cols = p.ntj * p.ntk;
-    // about four resident waves of workgroups over 256 CUs at 2 per CU
-    const int target = 2048;
+    // about four resident rounds of workgroups over 256 CUs
+    const int target = 4 * 256 * per_cu;
     int nseg = (target + cols - 1) / cols;
     if (nseg > g.L) nseg = g.L;
     if (nseg < 1) nseg = 1;
@@ -345,7 +344,36 @@ Launch plan(const Geo& g)
     p.remap = 1;
     const int padded = (p.nblocks + 7) / 8 * 8;
     p.grid = dim3(padded);
+    p.block = dim3(BK, BJ);
     return p;
+}
+
+template <int BJ, int BK, int MINW>
+void launch_variant(hipStream_t s, int stage, FusedArgs a, int per_cu)
+{
+    const Launch p = plan(a.g, BJ, BK, per_cu);
+    a.segL = p.segL;
+    a.nseg = p.nseg;
+    a.ntj = p.ntj;
+    a.ntk = p.ntk;
+    a.nblocks = p.nblocks;
+    a.xcd_remap = p.remap;
+    if (stage == 1) hipLaunchKernelGGL((k_fused<1, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_fused<2, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused<3, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
+}
+
+// Tile shape / occupancy variants: 0 = 16x16, 2 waves/SIMD (default);
+// 1 = 16x16, 1 wave/SIMD; 2 = 16x32, 2 waves/SIMD; 3 = 16x32, 1 wave/SIMD
+int fused_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("FSL_FUSED_VARIANT");
+        v = e ? atoi(e) : 0;
+        if (v < 0 || v > 3) v = 0;
+    }
+    return v;
 }
 
 }  // namespace
@@ -363,7 +391,7 @@ int fused_stage(fsl_ctx* ctx, int stage)
         rc = ctx_sgs(ctx);
         if (rc) return rc;
     }
-    FusedArgs a;
+    FusedArgs a{};
     for (int f = 0; f < 5; f++) {
         a.in.f[f] = in + f * v.g.G;
         a.base.f[f] = v.U + f * v.g.G;
@@ -376,18 +404,13 @@ int fused_stage(fsl_ctx* ctx, int stage)
     a.dtX = v.dtX;
     a.dtY = v.dtY;
     a.dtZ = v.dtZ;
-    const Launch p = plan(v.g);
-    a.segL = p.segL;
-    a.nseg = p.nseg;
-    a.ntj = p.ntj;
-    a.ntk = p.ntk;
-    a.nblocks = p.nblocks;
-    a.xcd_remap = p.remap;
-    const dim3 block(BK, BJ);
     ctx_prof(ctx, K_FUSED_XYZ, 0);
-    if (stage == 1) hipLaunchKernelGGL(k_fused<1>, p.grid, block, 0, v.stream, a);
-    else if (stage == 2) hipLaunchKernelGGL(k_fused<2>, p.grid, block, 0, v.stream, a);
-    else hipLaunchKernelGGL(k_fused<3>, p.grid, block, 0, v.stream, a);
+    switch (fused_variant()) {
+    case 1: launch_variant<16, 16, 1>(v.stream, stage, a, 1); break;
+    case 2: launch_variant<16, 32, 2>(v.stream, stage, a, 1); break;
+    case 3: launch_variant<16, 32, 1>(v.stream, stage, a, 1); break;
+    default: launch_variant<16, 16, 2>(v.stream, stage, a, 2); break;
+    }
     ctx_prof(ctx, K_FUSED_XYZ, 1);
     rc = ctx_check(ctx, "fused stage");
     if (rc) return rc;
