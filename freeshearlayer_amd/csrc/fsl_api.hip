@@ -44,6 +44,7 @@ struct fsl_ctx {
     double* faces;      // 10 (NX + NY + NZ), unfused path only
     double* mu;         // G
     double* sgs;        // scratch for the dynamic model
+    double* filt;       // scratch of fsl_filter (in, out and two pass buffers)
     double* u_in;       // HIG
     double* sums;       // partials + 5 results
     unsigned long long* red;
@@ -70,7 +71,7 @@ namespace {
 
 const char* kNames[K_NKERNELS] = {
     "ghost_fill", "recon", "iface_bc", "flux_x", "flux_y", "flux_z", "rk_update", "cfl_reduce",
-    "sgs_prims", "filter_y", "filter_z", "filter_x", "sgs_strain", "sgs_final", "fused_x",
+    "sgs_filter_a", "filter_y", "filter_z", "filter_x", "sgs_strain", "sgs_filter_b", "fused_x",
     "fused_y", "fused_z", "sums", "copy", "fused_stage"};
 
 int set_err(fsl_ctx* c, int code, const char* fmt, ...)
@@ -190,16 +191,20 @@ SgsBufs sgs_bufs(fsl_ctx* c)
     SgsBufs B;
     double* p = c->sgs;
     const long long G = c->g.G;
-    for (int a = 0; a < 9; a++) { B.prim[a] = p; p += G; }
-    for (int a = 0; a < 9; a++) { B.filt[a] = p; p += G; }
-    for (int a = 0; a < 9; a++) { B.t1[a] = p; p += G; }
-    for (int a = 0; a < 9; a++) { B.t2[a] = p; p += G; }
+    for (int a = 0; a < 3; a++) { B.uf[a] = p; p += G; }
     for (int a = 0; a < 6; a++) { B.Lij[a] = p; p += G; }
-    for (int a = 0; a < 6; a++) { B.Bij[a] = p; p += G; }
     for (int a = 0; a < 6; a++) { B.Aij[a] = p; p += G; }
-    for (int a = 0; a < 6; a++) { B.Af[a] = p; p += G; }
     B.mag = p;
     return B;
+}
+
+int ensure_filt(fsl_ctx* c)
+{
+    if (c->filt) return FSL_OK;
+    const size_t n = 4 * (size_t)c->g.G;
+    HIPCHK(c, hipMalloc(&c->filt, n * sizeof(double)));
+    c->bytes += n * sizeof(double);
+    return FSL_OK;
 }
 
 int check_launch(fsl_ctx* c, const char* what)
@@ -354,6 +359,7 @@ void fsl_destroy(fsl_ctx* c)
     (void)hipFree(c->faces);
     (void)hipFree(c->mu);
     (void)hipFree(c->sgs);
+    (void)hipFree(c->filt);
     (void)hipFree(c->u_in);
     (void)hipFree(c->sums);
     (void)hipFree(c->red);
@@ -653,15 +659,16 @@ int fsl_check_co_num(fsl_ctx* c, double* maxCo_out)
 
 int fsl_filter(fsl_ctx* c, const double* in, double* out, const double h[3])
 {
-    int rc = ensure_sgs(c);
+    int rc = ensure_filt(c);
     if (rc) return rc;
-    SgsBufs B = sgs_bufs(c);
-    const size_t nb = c->g.G * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(B.prim[0], in, nb, hipMemcpyHostToDevice, c->stream));
-    launch_filter(c->stream, B.prim[0], B.filt[0], B.t1[0], B.t2[0], c->g, h, prof_mark, c);
+    const long long G = c->g.G;
+    double *d_in = c->filt, *d_out = c->filt + G, *t1 = c->filt + 2 * G, *t2 = c->filt + 3 * G;
+    const size_t nb = G * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(d_in, in, nb, hipMemcpyHostToDevice, c->stream));
+    launch_filter(c->stream, d_in, d_out, t1, t2, c->g, h, prof_mark, c);
     rc = check_launch(c, "filter");
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(out, B.filt[0], nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, d_out, nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSL_OK;
 }

@@ -60,18 +60,14 @@ int sums_nblocks(const Geo& g);
 void launch_copy(hipStream_t s, const double* a, double* b, size_t n);
 
 // --- dynamic Smagorinsky (fsl_sgs.hip) ---
-// Scratch: 9 primitive/product arrays, 9 filtered, 2 filter temporaries per
-// filtered field, L (6), B (6), A (6), A filtered (6), |S|.
+// Intermediates of the tiled model: filtered velocities (zero outside the
+// interior and the i = 0 plane), Leonard stress (6 distinct components),
+// A = |S| S (6, ghost layer kept 0) and |S|.
 struct SgsBufs {
-    double* prim[9];    // u, v, w, uu, uv, uw, vv, vw, ww
-    double* filt[9];    // their box-filtered fields
-    double* t1[9];      // y-pass temporaries
-    double* t2[9];      // z-pass temporaries
-    double* Lij[6];     // Leonard stress (11, 12, 13, 22, 23, 33)
-    double* Bij[6];     // |S^| S^_ij
-    double* Aij[6];     // |S| S_ij, ghost layer kept 0
-    double* Af[6];      // filtered A
-    double* mag;        // |S|
+    double* uf[3];
+    double* Lij[6];     // 11, 12, 13, 22, 23, 33
+    double* Aij[6];
+    double* mag;
 };
 size_t sgs_scratch_arrays();   // number of cell arrays SgsBufs needs
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,

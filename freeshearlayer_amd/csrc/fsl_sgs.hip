@@ -6,7 +6,7 @@
 // (u v == v u in IEEE arithmetic), so 15 filters give the same result.  The
 // model reads the step-start state only (src/fluxes.c:53 passes U1..U4,
 // whose ghost layer is filled at stage 1 and unchanged until the end of
-// stage 3), so the context evaluates it once per step.
+// stage 3), so the fused path evaluates it once per step.
 //
 // Filter semantics reproduced exactly (src/turbulence.c:47-95): three
 // separable passes over index boxes that leave zeros outside them:
@@ -14,6 +14,17 @@
 //   z pass  t2 on i in [0,L], j in [0,H], k in [1,D]
 //   x pass  out on i in [1,L], j in [0,H], k in [0,D]; elsewhere out = t1
 // with each sum accumulated as ((0 + a h) + b h) + c h.
+//
+// Two tiled kernels do the whole model.  Each workgroup owns a 2 x 8 x 32
+// tile of interior cells and stages a one-cell-halo box in LDS:
+//   k_sgs_a  U -> u, v, w on the box; per field (u, v, w and the six
+//            products) the y, z and x passes on the box; Leonard stress,
+//            |S| and A = |S| S at the tile's cells; writes u^, L, A, |S|
+//            (and u^ on the i = 0 plane, which keeps its y-pass value).
+//   k_sgs_b  A^ by the same three passes, S^ from u^, B = |S^| S^,
+//            M = Delta^2 (4 B - A^), Cd and mu_SGS.
+// Every intermediate value is computed by the same operations on the same
+// inputs as the pass-by-pass form, so mu_SGS is bit-identical to it.
 #include "fsl_internal.hpp"
 
 namespace fsl {
@@ -30,6 +41,9 @@ struct Ptr9 {
 struct CPtr9 {
     const double* p[9];
 };
+struct H3 {
+    double h[3];
+};
 
 __device__ __forceinline__ void decode(const Geo& g, long long t, int& i, int& j, int& k)
 {
@@ -38,29 +52,7 @@ __device__ __forceinline__ void decode(const Geo& g, long long t, int& i, int& j
     i = (int)(t / g.si);
 }
 
-// src/turbulence.c:196-230: primitives and the six distinct products on the
-// whole ghosted box
-__global__ void __launch_bounds__(kBlock) k_sgs_prims(CState5 U, Ptr9 out, long long G)
-{
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= G) return;
-    const double rr = 1. / U.f[0][t];
-    const double u = U.f[1][t] * rr, v = U.f[2][t] * rr, w = U.f[3][t] * rr;
-    out.p[0][t] = u;
-    out.p[1][t] = v;
-    out.p[2][t] = w;
-    out.p[3][t] = u * u;
-    out.p[4][t] = u * v;
-    out.p[5][t] = u * w;
-    out.p[6][t] = v * v;
-    out.p[7][t] = v * w;
-    out.p[8][t] = w * w;
-}
-
-struct H3 {
-    double h[3];
-};
-
+// ---- pass-by-pass filter (fsl_filter: any cell array, any kernel h) ----
 __global__ void __launch_bounds__(kBlock) k_filt_y(CPtr9 in, Ptr9 t1, Geo g, H3 w)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,26 +112,117 @@ __global__ void __launch_bounds__(kBlock) k_filt_x(CPtr9 t2, CPtr9 t1, Ptr9 out,
     out.p[a][t] = r;
 }
 
-// src/turbulence.c:250-382: Leonard stress, strain rates of the resolved and
-// the filtered velocity, A = |S| S and B = |S^| S^ (interior cells)
-struct StrainArgs {
-    const double* prim[3];   // u, v, w
-    const double* filt[9];   // u^, v^, w^, (uu)^, (uv)^, (uw)^, (vv)^, (vw)^, (ww)^
-    double* Lij[6];
-    double* Bij[6];
-    double* Aij[6];
-    double* mag;
+// ---- tiled model ----
+constexpr int TI = 2, TJ = 8, TK = 32;                 // interior cells per workgroup
+constexpr int BI = TI + 2, BJ = TJ + 2, BK = TK + 2;   // box with a one-cell halo
+constexpr int NB = BI * BJ * BK;
+constexpr int NT1 = BI * TJ * BK;                      // y-pass values on the box
+constexpr int NT2 = BI * TJ * TK;                      // z-pass values on the box
+constexpr int NC = TI * TJ * TK;                       // tile cells
+constexpr int CPT = NC / kBlock;                       // cells per thread
+static_assert(NC % kBlock == 0, "tile must divide the workgroup");
+
+struct Tile {
+    int i0, j0, k0;
 };
 
-__device__ __forceinline__ void strain9(const double* const vel[3], long long c, const Geo& g,
-                                        const Phys& ph, double S[9])
+__device__ __forceinline__ Tile tile_of(const Geo& g)
+{
+    const int ntk = (g.D + TK - 1) / TK, ntj = (g.H + TJ - 1) / TJ;
+    int b = blockIdx.x;
+    const int tk = b % ntk;
+    b /= ntk;
+    const int tj = b % ntj;
+    const int ti = b / ntj;
+    return Tile{1 + ti * TI, 1 + tj * TJ, 1 + tk * TK};
+}
+
+__device__ __forceinline__ int bidx(int bi, int bj, int bk) { return (bi * BJ + bj) * BK + bk; }
+
+// field a of {u, v, w, uu, uv, uw, vv, vw, ww} at box cell b
+__device__ __forceinline__ double field9(const double (*s)[NB], int a, int b)
+{
+    switch (a) {
+    case 0: return s[0][b];
+    case 1: return s[1][b];
+    case 2: return s[2][b];
+    case 3: return s[0][b] * s[0][b];
+    case 4: return s[0][b] * s[1][b];
+    case 5: return s[0][b] * s[2][b];
+    case 6: return s[1][b] * s[1][b];
+    case 7: return s[1][b] * s[2][b];
+    default: return s[2][b] * s[2][b];
+    }
+}
+
+// The three passes of the box filter for one field whose box values come
+// from src(b).  out[q] receives the filtered value at this thread's tile
+// cells; t1i0 (optional) the y-pass value on the i = 0 plane.
+template <class Src>
+__device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, double h, Src src, double* s_t1,
+                                           double* s_t2, double out[CPT], double* t1i0_out, int tid)
+{
+    for (int e = tid; e < NT1; e += kBlock) {
+        const int bk = e % BK, tj = (e / BK) % TJ, bi = e / (BK * TJ);
+        const int gi = T.i0 - 1 + bi, gj = T.j0 + tj, gk = T.k0 - 1 + bk;
+        double r = 0.;
+        if (gi <= g.L && gj <= g.H && gk <= g.D) {
+            double tmp = 0.;
+            tmp += src(bidx(bi, tj, bk)) * h;
+            tmp += src(bidx(bi, tj + 1, bk)) * h;
+            tmp += src(bidx(bi, tj + 2, bk)) * h;
+            r = tmp;
+        }
+        s_t1[e] = r;
+    }
+    __syncthreads();
+    for (int e = tid; e < NT2; e += kBlock) {
+        const int tk = e % TK, tj = (e / TK) % TJ, bi = e / (TK * TJ);
+        const int gi = T.i0 - 1 + bi, gj = T.j0 + tj, gk = T.k0 + tk;
+        double r = 0.;
+        if (gi <= g.L && gj <= g.H && gk <= g.D) {
+            const int b1 = (bi * TJ + tj) * BK + tk;
+            double tmp = 0.;
+            tmp += s_t1[b1] * h;
+            tmp += s_t1[b1 + 1] * h;
+            tmp += s_t1[b1 + 2] * h;
+            r = tmp;
+        }
+        s_t2[e] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int e = tid + q * kBlock;
+        const int tk = e % TK, tj = (e / TK) % TJ, ti = e / (TK * TJ);
+        const int b2 = (ti * TJ + tj) * TK + tk;
+        double tmp = 0.;
+        tmp += s_t2[b2] * h;
+        tmp += s_t2[b2 + TJ * TK] * h;
+        tmp += s_t2[b2 + 2 * TJ * TK] * h;
+        out[q] = tmp;
+    }
+    if (t1i0_out && T.i0 == 1) {
+        // u^ on the i = 0 plane keeps the y-pass value (the x pass starts at i = 1)
+        for (int e = tid; e < TJ * TK; e += kBlock) {
+            const int tk = e % TK, tj = e / TK;
+            const int gj = T.j0 + tj, gk = T.k0 + tk;
+            if (gj <= g.H && gk <= g.D)
+                t1i0_out[(long long)gj * g.sj + gk] = s_t1[(0 * TJ + tj) * BK + tk + 1];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void strain_box(const double (*s)[NB], int b, const Phys& ph, double S[9])
 {
     double gr[3][3];
+    constexpr int sI = BJ * BK, sJ = BK;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        gr[r][0] = (vel[r][c + g.si] - vel[r][c - g.si]) * ph.r2del[0];
-        gr[r][1] = (vel[r][c + g.sj] - vel[r][c - g.sj]) * ph.r2del[1];
-        gr[r][2] = (vel[r][c + 1] - vel[r][c - 1]) * ph.r2del[2];
+        gr[r][0] = (s[r][b + sI] - s[r][b - sI]) * ph.r2del[0];
+        gr[r][1] = (s[r][b + sJ] - s[r][b - sJ]) * ph.r2del[1];
+        gr[r][2] = (s[r][b + 1] - s[r][b - 1]) * ph.r2del[2];
     }
     S[0] = gr[0][0];
     S[1] = 0.5 * (gr[0][1] + gr[1][0]);
@@ -158,77 +241,150 @@ __device__ __forceinline__ double mag9(const double S[9])
                      S[6] * S[6] + S[7] * S[7] + S[8] * S[8]));
 }
 
-__global__ void __launch_bounds__(kBlock) k_sgs_strain(StrainArgs a, Geo g, Phys ph)
+struct SgsA {
+    CState5 U;
+    double* uf[3];
+    double* Lij[6];
+    double* Aij[6];
+    double* mag;
+};
+
+__global__ void __launch_bounds__(kBlock) k_sgs_a(SgsA a, Geo g, Phys ph)
 {
-    const long long n = (long long)g.L * g.H * g.D;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int k = (int)(t % g.D) + 1;
-    const int j = (int)((t / g.D) % g.H) + 1;
-    const int i = (int)(t / ((long long)g.D * g.H)) + 1;
-    const long long c = cidx(g, i, j, k);
-    const double uf = a.filt[0][c], vf = a.filt[1][c], wf = a.filt[2][c];
-    a.Lij[0][c] = a.filt[3][c] - uf * uf;
-    a.Lij[1][c] = a.filt[4][c] - uf * vf;
-    a.Lij[2][c] = a.filt[5][c] - uf * wf;
-    a.Lij[3][c] = a.filt[6][c] - vf * vf;
-    a.Lij[4][c] = a.filt[7][c] - vf * wf;
-    a.Lij[5][c] = a.filt[8][c] - wf * wf;
-    double S[9], Sf[9];
-    const double* vel[3] = {a.prim[0], a.prim[1], a.prim[2]};
-    const double* velf[3] = {a.filt[0], a.filt[1], a.filt[2]};
-    strain9(vel, c, g, ph, S);
-    const double Sm = mag9(S);
-    a.mag[c] = Sm;
-    strain9(velf, c, g, ph, Sf);
-    const double Sfm = mag9(Sf);
-    const int uniq[6] = {0, 1, 2, 4, 5, 8};   // 11, 12, 13, 22, 23, 33
+    __shared__ double s_u[3][NB];
+    __shared__ double s_t1[NT1];
+    __shared__ double s_t2[NT2];
+    const Tile T = tile_of(g);
+    const int tid = threadIdx.x;
+    // u, v, w = (rho u, rho v, rho w) * (1 / rho) on the box (src/turbulence.c:197-209)
+    for (int e = tid; e < NB; e += kBlock) {
+        const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
+        const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
+        double u = 0., v = 0., w = 0.;
+        if (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) {
+            const long long c = cidx(g, gi, gj, gk);
+            const double rr = 1. / a.U.f[0][c];
+            u = a.U.f[1][c] * rr;
+            v = a.U.f[2][c] * rr;
+            w = a.U.f[3][c] * rr;
+        }
+        s_u[0][e] = u;
+        s_u[1][e] = v;
+        s_u[2][e] = w;
+    }
+    __syncthreads();
+    const double h = 1. / 3.;
+    double fil[9][CPT];
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-        a.Bij[q][c] = Sfm * Sf[uniq[q]];
-        a.Aij[q][c] = Sm * S[uniq[q]];
+    for (int f = 0; f < 9; f++) {
+        box_filter(g, T, h, [&](int b) { return field9(s_u, f, b); }, s_t1, s_t2, fil[f],
+                   f < 3 ? a.uf[f] : nullptr, tid);
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int e = tid + q * kBlock;
+        const int tk = e % TK, tj = (e / TK) % TJ, ti = e / (TK * TJ);
+        const int gi = T.i0 + ti, gj = T.j0 + tj, gk = T.k0 + tk;
+        if (gi > g.L || gj > g.H || gk > g.D) continue;
+        const long long c = cidx(g, gi, gj, gk);
+        const double uf = fil[0][q], vf = fil[1][q], wf = fil[2][q];
+        a.uf[0][c] = uf;
+        a.uf[1][c] = vf;
+        a.uf[2][c] = wf;
+        a.Lij[0][c] = fil[3][q] - uf * uf;
+        a.Lij[1][c] = fil[4][q] - uf * vf;
+        a.Lij[2][c] = fil[5][q] - uf * wf;
+        a.Lij[3][c] = fil[6][q] - vf * vf;
+        a.Lij[4][c] = fil[7][q] - vf * wf;
+        a.Lij[5][c] = fil[8][q] - wf * wf;
+        double S[9];
+        strain_box(s_u, bidx(ti + 1, tj + 1, tk + 1), ph, S);
+        const double Sm = mag9(S);
+        a.mag[c] = Sm;
+        const int uniq[6] = {0, 1, 2, 4, 5, 8};   // 11, 12, 13, 22, 23, 33
+#pragma unroll
+        for (int m = 0; m < 6; m++) a.Aij[m][c] = Sm * S[uniq[m]];
     }
 }
 
-// src/turbulence.c:398-445: M = Delta^2 (4 B - A^), deviatoric L, Cd, mu_SGS
-struct FinalArgs {
+struct SgsB {
     const double* rho;
+    const double* uf[3];
     const double* Lij[6];
-    const double* Bij[6];
-    const double* Af[6];
+    const double* Aij[6];
     const double* mag;
     double* mu;
 };
 
-__global__ void __launch_bounds__(kBlock) k_sgs_final(FinalArgs a, Geo g, Phys ph)
+__global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
 {
-    const long long n = (long long)g.L * g.H * g.D;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int k = (int)(t % g.D) + 1;
-    const int j = (int)((t / g.D) % g.H) + 1;
-    const int i = (int)(t / ((long long)g.D * g.H)) + 1;
-    const long long c = cidx(g, i, j, k);
-    // 3x3 from the six distinct components: 11 12 13 / 21=12 22 23 / 31=13 32=23 33
-    const int map9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-    double M[9], Lm[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-        M[q] = ph.DD * (4 * a.Bij[map9[q]][c] - a.Af[map9[q]][c]);
-        Lm[q] = a.Lij[map9[q]][c];
+    __shared__ double s_uf[3][NB];
+    __shared__ double s_a[NB];
+    __shared__ double s_t1[NT1];
+    __shared__ double s_t2[NT2];
+    const Tile T = tile_of(g);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB; e += kBlock) {
+        const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
+        const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
+        double u = 0., v = 0., w = 0.;
+        if (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) {
+            const long long c = cidx(g, gi, gj, gk);
+            u = a.uf[0][c];
+            v = a.uf[1][c];
+            w = a.uf[2][c];
+        }
+        s_uf[0][e] = u;
+        s_uf[1][e] = v;
+        s_uf[2][e] = w;
     }
-    const double tr = -(1.0 / 3.0) * (Lm[0] + Lm[4] + Lm[8]);
-    Lm[0] += tr;
-    Lm[4] += tr;
-    Lm[8] += tr;
-    const double LLMM = (Lm[0] * M[0] + Lm[1] * M[1] + Lm[2] * M[2] + Lm[3] * M[3] + Lm[4] * M[4] +
-                         Lm[5] * M[5] + Lm[6] * M[6] + Lm[7] * M[7] + Lm[8] * M[8]);
-    const double MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
-                         M[5] * M[5] + M[6] * M[6] + M[7] * M[7] + M[8] * M[8]);
-    double Cd = -0.5 * (LLMM / (MMMM + kSmall));
-    if (Cd > 0.15) Cd = 0.15;
-    else if (Cd < 0.0) Cd = 0.0;
-    a.mu[c] = a.rho[c] * Cd * ph.DD * a.mag[c];
+    const double h = 1. / 3.;
+    double Af[6][CPT];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        __syncthreads();   // previous users of s_a are done
+        for (int e = tid; e < NB; e += kBlock) {
+            const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
+            const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
+            s_a[e] = (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) ? a.Aij[m][cidx(g, gi, gj, gk)] : 0.;
+        }
+        __syncthreads();
+        box_filter(g, T, h, [&](int b) { return s_a[b]; }, s_t1, s_t2, Af[m], nullptr, tid);
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int e = tid + q * kBlock;
+        const int tk = e % TK, tj = (e / TK) % TJ, ti = e / (TK * TJ);
+        const int gi = T.i0 + ti, gj = T.j0 + tj, gk = T.k0 + tk;
+        if (gi > g.L || gj > g.H || gk > g.D) continue;
+        const long long c = cidx(g, gi, gj, gk);
+        double Sf[9];
+        strain_box(s_uf, bidx(ti + 1, tj + 1, tk + 1), ph, Sf);
+        const double Sfm = mag9(Sf);
+        // 3x3 from the six distinct components: 11 12 13 / 21=12 22 23 / 31=13 32=23 33
+        const int map9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        const int uniq[6] = {0, 1, 2, 4, 5, 8};
+        double B6[6], M[9], Lm[9];
+#pragma unroll
+        for (int m = 0; m < 6; m++) B6[m] = Sfm * Sf[uniq[m]];
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            M[r] = ph.DD * (4 * B6[map9[r]] - Af[map9[r]][q]);
+            Lm[r] = a.Lij[map9[r]][c];
+        }
+        const double tr = -(1.0 / 3.0) * (Lm[0] + Lm[4] + Lm[8]);
+        Lm[0] += tr;
+        Lm[4] += tr;
+        Lm[8] += tr;
+        const double LLMM = (Lm[0] * M[0] + Lm[1] * M[1] + Lm[2] * M[2] + Lm[3] * M[3] + Lm[4] * M[4] +
+                             Lm[5] * M[5] + Lm[6] * M[6] + Lm[7] * M[7] + Lm[8] * M[8]);
+        const double MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
+                             M[5] * M[5] + M[6] * M[6] + M[7] * M[7] + M[8] * M[8]);
+        double Cd = -0.5 * (LLMM / (MMMM + kSmall));
+        if (Cd > 0.15) Cd = 0.15;
+        else if (Cd < 0.0) Cd = 0.0;
+        a.mu[c] = a.rho[c] * Cd * ph.DD * a.mag[c];
+    }
 }
 
 void run_filters(hipStream_t s, int nf, const double* const* in, double* const* out, double* const* t1,
@@ -260,7 +416,7 @@ void run_filters(hipStream_t s, int nf, const double* const* in, double* const* 
 
 }  // namespace
 
-size_t sgs_scratch_arrays() { return 9 * 4 + 6 * 4 + 1; }
+size_t sgs_scratch_arrays() { return 3 + 6 + 6 + 1; }
 
 void launch_filter(hipStream_t s, const double* in, double* out, double* t1, double* t2, const Geo& g,
                    const double h[3], void (*prof)(void*, int, int), void* pa)
@@ -271,47 +427,29 @@ void launch_filter(hipStream_t s, const double* in, double* out, double* t1, dou
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
                 void (*prof)(void*, int, int), void* pa)
 {
-    // src/turbulence.c:113: box kernel, 1./3. stored as real
-    const double box[3] = {1. / 3., 1. / 3., 1. / 3.};
-    Ptr9 prim{};
-    for (int a = 0; a < 9; a++) prim.p[a] = B.prim[a];
+    const int ntiles = ((g.L + TI - 1) / TI) * ((g.H + TJ - 1) / TJ) * ((g.D + TK - 1) / TK);
+    SgsA A{};
+    A.U = U;
+    for (int r = 0; r < 3; r++) A.uf[r] = B.uf[r];
+    for (int m = 0; m < 6; m++) {
+        A.Lij[m] = B.Lij[m];
+        A.Aij[m] = B.Aij[m];
+    }
+    A.mag = B.mag;
     if (prof) prof(pa, K_SGS_PRIM, 0);
-    hipLaunchKernelGGL(k_sgs_prims, dim3(nblk(g.G)), dim3(kBlock), 0, s, U, prim, g.G);
+    hipLaunchKernelGGL(k_sgs_a, dim3(ntiles), dim3(kBlock), 0, s, A, g, ph);
     if (prof) prof(pa, K_SGS_PRIM, 1);
-
-    const double* in9[9];
-    for (int a = 0; a < 9; a++) in9[a] = B.prim[a];
-    run_filters(s, 9, in9, B.filt, B.t1, B.t2, g, box, prof, pa);
-
-    StrainArgs sa{};
-    for (int r = 0; r < 3; r++) sa.prim[r] = B.prim[r];
-    for (int a = 0; a < 9; a++) sa.filt[a] = B.filt[a];
-    for (int q = 0; q < 6; q++) {
-        sa.Lij[q] = B.Lij[q];
-        sa.Bij[q] = B.Bij[q];
-        sa.Aij[q] = B.Aij[q];
+    SgsB Bk{};
+    Bk.rho = U.f[0];
+    for (int r = 0; r < 3; r++) Bk.uf[r] = B.uf[r];
+    for (int m = 0; m < 6; m++) {
+        Bk.Lij[m] = B.Lij[m];
+        Bk.Aij[m] = B.Aij[m];
     }
-    sa.mag = B.mag;
-    const long long n = (long long)g.L * g.H * g.D;
-    if (prof) prof(pa, K_SGS_STRAIN, 0);
-    hipLaunchKernelGGL(k_sgs_strain, dim3(nblk(n)), dim3(kBlock), 0, s, sa, g, ph);
-    if (prof) prof(pa, K_SGS_STRAIN, 1);
-
-    const double* a6[6];
-    for (int q = 0; q < 6; q++) a6[q] = B.Aij[q];
-    run_filters(s, 6, a6, B.Af, B.t1, B.t2, g, box, prof, pa);
-
-    FinalArgs fa{};
-    fa.rho = U.f[0];
-    for (int q = 0; q < 6; q++) {
-        fa.Lij[q] = B.Lij[q];
-        fa.Bij[q] = B.Bij[q];
-        fa.Af[q] = B.Af[q];
-    }
-    fa.mag = B.mag;
-    fa.mu = mu;
+    Bk.mag = B.mag;
+    Bk.mu = mu;
     if (prof) prof(pa, K_SGS_FINAL, 0);
-    hipLaunchKernelGGL(k_sgs_final, dim3(nblk(n)), dim3(kBlock), 0, s, fa, g, ph);
+    hipLaunchKernelGGL(k_sgs_b, dim3(ntiles), dim3(kBlock), 0, s, Bk, g, ph);
     if (prof) prof(pa, K_SGS_FINAL, 1);
 }
 
