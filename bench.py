@@ -52,9 +52,10 @@ def algorithmic_bytes(kernel, L, H, D, sgs):
         "flux_z": 10 * 8 * nz + 5 * 8 * G + (8 * G if sgs else 0) + 5 * 8 * nz,
         # read 15 flux arrays, U and Up, write one state
         "rk_update": 5 * 8 * (nx + ny + nz) + 3 * 5 * 8 * N,
-        # fused stage: read stage state (5) and step state (5), mu, write 5;
-        # stage 1 reads the step state once (it is the stage state)
-        "fused_stage": (5 * 8 * N * (2 if True else 1) + 5 * 8 * N + (8 * N if sgs else 0)),
+        # fused stage, averaged over the three stages: read the stage state
+        # (40 B/cell), the step state in stages 2-3 (40 B/cell), mu_SGS (8) and
+        # write the new state (40): (88 + 128 + 128) / 3 dynamic, (80+120+120)/3 constant
+        "fused_stage": N * ((40 + 40 + (8 if sgs else 0)) + 2 * (40 + 40 + 40 + (8 if sgs else 0))) / 3.0,
         "filter_x": 2 * 8 * G, "filter_y": 2 * 8 * G, "filter_z": 2 * 8 * G,
     }
     return per.get(kernel)
