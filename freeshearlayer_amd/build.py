@@ -38,7 +38,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-std=c99", "-Wall", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE]
 
-HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip"]
+HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip"]
 HOST_SOURCES = ["fsl_host.c", "fsl_driver.c"]
 
 

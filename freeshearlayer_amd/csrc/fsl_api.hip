@@ -752,17 +752,6 @@ int fsl_copy_kernel(fsl_ctx* c, size_t n, int reps)
 
 // --- accessors for the fused path (fsl_fused.hip) ---------------------
 namespace fsl {
-struct CtxView {
-    hipStream_t stream;
-    Geo g;
-    Phys ph;
-    Bnd bnd;
-    double *U, *Up, *Uq, *mu;
-    double dtX, dtY, dtZ;
-    int cur_is_p;
-    int sgs_model;
-    int precision;
-};
 
 CtxView ctx_view(fsl_ctx* c)
 {

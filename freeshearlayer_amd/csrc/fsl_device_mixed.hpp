@@ -1,0 +1,223 @@
+// fsl_device_mixed.hpp -- building blocks of the mixed-precision path.
+//
+// Same algorithm as fsl_device.hpp (src/reconstruction.c, src/fluxes.c), with
+// the characteristic transform, limiter, Riemann solver and viscous/SGS flux
+// in FP32 and these kept in FP64:
+//   * the conserved state and every difference taken from it (the
+//     reconstruction's forward/backward differences are formed in FP64 and
+//     rounded once),
+//   * the face states (cell value in FP64 plus an FP32 correction),
+//   * pressure recovered from total energy, p = (K-1)(E - rho|u|^2/2): E is
+//     FP64 and only the kinetic part is FP32, so the cancellation at low Mach
+//     costs no FP32 digits,
+//   * the Runge-Kutta accumulation (fsl_fused_mixed.hip).
+// FP32 reciprocals and square roots use the hardware approximations
+// (v_rcp_f32, v_sqrt_f32, about 1 ulp) instead of correctly rounded
+// sequences; contraction into FMA is allowed in these functions.
+#pragma once
+
+#include "fsl_device.hpp"
+
+namespace fsl {
+namespace mx {
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// pressure from FP64 total energy and an FP32 kinetic energy
+__device__ __forceinline__ float pressure(double E, float ke) { return (float)((E - (double)ke) * kK_1); }
+
+struct PrimF {
+    float R, vel[3], P, C, aa, ll, ee, gg, hh, nn, S11;
+    float vhh[3], vee[3];
+};
+
+__device__ __forceinline__ void cell_prims(const double q[5], PrimF& c)
+{
+#pragma clang fp contract(fast)
+    const float r = (float)q[0];
+    c.R = rcp(r);
+    const float m0 = (float)q[1], m1 = (float)q[2], m2 = (float)q[3];
+    c.vel[0] = m0 * c.R;
+    c.vel[1] = m1 * c.R;
+    c.vel[2] = m2 * c.R;
+    const float ff = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
+    c.P = pressure(q[4], 0.5f * r * ff);
+    c.C = fsqrt(1.4f * c.P * c.R);
+    c.aa = 0.2f * ff;
+    c.ll = 0.5f * rcp(c.C);
+    c.ee = c.ll * rcp(c.C);
+    c.gg = -ff * c.ee;
+    c.hh = -(c.ee + c.ee);
+    c.S11 = c.aa - c.C * c.C;
+#pragma unroll
+    for (int r2 = 0; r2 < 3; r2++) {
+        c.vhh[r2] = c.vel[r2] * c.hh;
+        c.vee[r2] = c.vel[r2] * c.ee;
+    }
+    c.nn = -0.5f * c.gg;
+}
+
+template <int N, int T1, int T2>
+__device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], const double qp[5],
+                                          const double qm[5], bool quirk, double plus[5], double minus[5])
+{
+#pragma clang fp contract(fast)
+    const int perm[5] = {0, N, T1, T2, 4};
+    float m[5], w[5], M[5], W[5], Mh[5], Wh[5], kv[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        m[r] = (float)(qp[perm[r]] - q[perm[r]]);
+        w[r] = (float)(q[perm[r]] - qm[perm[r]]);
+    }
+    const float bb = -0.4f * c.vel[N - 1];
+    const float cc = -0.4f * c.vel[T1 - 1];
+    const float dd = -0.4f * c.vel[T2 - 1];
+    const float mm = c.vel[N - 1] * c.ll;
+    float kk = c.C * c.vel[N - 1];
+    const float S41 = c.aa - kk, S51 = c.aa + kk;
+    const float S12 = bb, S42 = bb + c.C, S52 = bb - c.C;
+    float k2, k3;
+    kk = 0.4f * m[4]; k2 = cc * m[2]; k3 = dd * m[3];
+    M[0] = c.S11 * m[0] + S12 * m[1] + k2 + k3 + kk;
+    M[1] = -c.vel[T1 - 1] * m[0] + m[2];
+    M[2] = -c.vel[T2 - 1] * m[0] + m[3];
+    M[3] = S41 * m[0] + S42 * m[1] + k2 + k3 + kk;
+    M[4] = S51 * m[0] + S52 * m[1] + k2 + k3 + kk;
+    kk = 0.4f * w[4]; k2 = cc * w[2]; k3 = dd * w[3];
+    W[0] = c.S11 * w[0] + S12 * w[1] + k2 + k3 + kk;
+    W[1] = -c.vel[T1 - 1] * w[0] + w[2];
+    W[2] = -c.vel[T2 - 1] * w[0] + w[3];
+    W[3] = S41 * w[0] + S42 * w[1] + k2 + k3 + kk;
+    W[4] = S51 * w[0] + S52 * w[1] + k2 + k3 + kk;
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        Mh[r] = minmod(M[r], W[r]);
+        Wh[r] = minmod(W[r], M[r]);
+    }
+    const float S24 = c.vee[N - 1] + c.ll, S25 = c.vee[N - 1] - c.ll;
+    const float S5 = c.nn + 1.25f;
+    const float S54 = S5 + mm, S55 = S5 - mm;
+    const float ee_t2 = quirk ? c.vee[N - 1] : c.vee[T2 - 1];
+    const float C1f = (float)kC1, C2f = (float)kC2;
+    float d[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) kv[r] = Mh[r] * C1f + Wh[r] * C2f;
+    kk = kv[3] + kv[4];
+    d[0] = c.hh * kv[0] + c.ee * kk;
+    d[N] = c.vhh[N - 1] * kv[0] + S24 * kv[3] + S25 * kv[4];
+    d[T1] = c.vhh[T1 - 1] * kv[0] + kv[1] + c.vee[T1 - 1] * kk;
+    d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
+    d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
+#pragma unroll
+    for (int r = 0; r < 5; r++) plus[r] = q[r] + (double)d[r];
+#pragma unroll
+    for (int r = 0; r < 5; r++) kv[r] = Mh[r] * C2f + Wh[r] * C1f;
+    kk = kv[3] + kv[4];
+    d[0] = c.hh * kv[0] + c.ee * kk;
+    d[N] = c.vhh[N - 1] * kv[0] + S24 * kv[3] + S25 * kv[4];
+    d[T1] = c.vhh[T1 - 1] * kv[0] + kv[1] + c.vee[T1 - 1] * kk;
+    d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
+    d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
+#pragma unroll
+    for (int r = 0; r < 5; r++) minus[r] = q[r] - (double)d[r];
+}
+
+template <int N>
+__device__ __forceinline__ void riemann(const double l[5], const double r[5], float& Ro, float vel[3], float& Po)
+{
+#pragma clang fp contract(fast)
+    float lv[3], rv[3];
+    const float lR = (float)l[0], rR = (float)r[0];
+    const float lRi = rcp(lR), rRi = rcp(rR);
+    lv[0] = (float)l[1] * lRi; lv[1] = (float)l[2] * lRi; lv[2] = (float)l[3] * lRi;
+    rv[0] = (float)r[1] * rRi; rv[1] = (float)r[2] * rRi; rv[2] = (float)r[3] * rRi;
+    const float lP = pressure(l[4], 0.5f * lR * (lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]));
+    const float rP = pressure(r[4], 0.5f * rR * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]));
+    const float lC = fsqrt(1.4f * lP * lRi), rC = fsqrt(1.4f * rP * rRi);
+    const float ljo = -0.4f * lP, rjo = -0.4f * rP;
+    const float lRU = lP * rcp(lR * lC), rRU = rP * rcp(rR * rC);
+    const float ljp = lv[N] + lRU, ljm = lv[N] - lRU;
+    const float rjp = rv[N] + rRU, rjm = rv[N] - rRU;
+    float Un = 0.5f * (lv[N] + rv[N]);
+    const float Cm = 0.5f * (lC + rC);
+    const float C_p = Un + Cm, C_m = Un - Cm, C_o = Un;
+    float al_p, al_m, al_o, J_p, J_m, J_o, a;
+    if (C_p > 0.f) { al_p = 0.2f * (ljm - ljp) * rcp(ljo); J_p = ljp; }
+    else           { al_p = 0.2f * (rjm - rjp) * rcp(rjo); J_p = rjp; }
+    if (C_m > 0.f) { al_m = 0.2f * (ljp - ljm) * rcp(ljo); J_m = ljm; }
+    else           { al_m = 0.2f * (rjp - rjm) * rcp(rjo); J_m = rjm; }
+    if (C_o > 0.f) { a = 0.7f * (ljp - ljm); al_o = -a * a; J_o = ljo; }
+    else           { a = 0.7f * (rjp - rjm); al_o = -a * a; J_o = rjo; }
+    const float P = (J_p - J_m) * rcp(al_p - al_m);
+    Un = J_p - al_p * P;
+    Ro = (J_o - P) * rcp(al_o);
+    const bool left = Un > 0.f;
+#pragma unroll
+    for (int t = 0; t < 3; t++) vel[t] = left ? lv[t] : rv[t];
+    vel[N] = Un;
+    Po = P;
+}
+
+// cell velocity, temperature (FP64 pressure recovery) and 1/rho velocity
+__device__ __forceinline__ void cell_visc(const double q[5], float v[3], float& Tc)
+{
+#pragma clang fp contract(fast)
+    const float r = (float)q[0];
+    const float ri = rcp(r);
+    v[0] = (float)q[1] * ri; v[1] = (float)q[2] * ri; v[2] = (float)q[3] * ri;
+    const float P = pressure(q[4], 0.5f * r * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    Tc = P * rcp((float)kRair * r);
+}
+
+template <int DIR>
+__device__ __forceinline__ void face_flux(const Phys& ph, float R, const float vel[3], float P, const float vf[3],
+                                          const float vb[3], float Tf, float Tb, float Rf, float Rb,
+                                          const float g[3][3], float mu_dyn, float F[5])
+{
+#pragma clang fp contract(fast)
+    const float uu = 0.5f * (vf[0] + vb[0]);
+    const float vv = 0.5f * (vf[1] + vb[1]);
+    const float ww = 0.5f * (vf[2] + vb[2]);
+    float mu_T;
+    if (ph.dynamic) {
+        mu_T = mu_dyn;
+    } else {
+        const float rr = 0.5f * (Rf + Rb);
+        const float S12 = 0.5f * (g[0][1] + g[1][0]);
+        const float S13 = 0.5f * (g[0][2] + g[2][0]);
+        const float S23 = 0.5f * (g[1][2] + g[2][1]);
+        const float S = fsqrt(2.f * (g[0][0] * g[0][0] + g[1][1] * g[1][1] + g[2][2] * g[2][2] +
+                                     2.f * (S12 * S12 + S13 * S13 + S23 * S23)));
+        mu_T = rr * (float)ph.CsDD * S;
+    }
+    const float mu_E = (float)ph.mu_L + mu_T;
+    const float t3 = (float)kTwo3;
+    float s0, s1, s2;
+    if (DIR == 0) {
+        s0 = t3 * mu_E * (g[0][0] + g[0][0] - g[1][1] - g[2][2]);
+        s1 = mu_E * (g[0][1] + g[1][0]);
+        s2 = mu_E * (g[0][2] + g[2][0]);
+    } else if (DIR == 1) {
+        s0 = mu_E * (g[1][0] + g[0][1]);
+        s1 = t3 * mu_E * (g[1][1] + g[1][1] - g[0][0] - g[2][2]);
+        s2 = mu_E * (g[1][2] + g[2][1]);
+    } else {
+        s0 = mu_E * (g[2][0] + g[0][2]);
+        s1 = mu_E * (g[2][1] + g[1][2]);
+        s2 = t3 * mu_E * (g[2][2] + g[2][2] - g[0][0] - g[1][1]);
+    }
+    const float lambda_E = (float)ph.lambda_L + mu_T * (float)ph.cp_Pr_T;
+    const float q = -lambda_E * (Tf - Tb) * (float)ph.rdel[DIR];
+    const float RU = R * vel[DIR];
+    F[0] = RU;
+    F[1] = RU * vel[0] - s0;
+    F[2] = RU * vel[1] - s1;
+    F[3] = RU * vel[2] - s2;
+    F[1 + DIR] += P;
+    F[4] = vel[DIR] * (3.5f * P + 0.5f * R * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) - uu * s0 -
+           vv * s1 - ww * s2 + q;
+}
+
+}  // namespace mx
+}  // namespace fsl

@@ -26,17 +26,6 @@
 
 namespace fsl {
 
-struct CtxView {
-    hipStream_t stream;
-    Geo g;
-    Phys ph;
-    Bnd bnd;
-    double *U, *Up, *Uq, *mu;
-    double dtX, dtY, dtZ;
-    int cur_is_p;
-    int sgs_model;
-    int precision;
-};
 CtxView ctx_view(fsl_ctx* c);
 void ctx_prof(fsl_ctx* c, int id, int end);
 int ctx_check(fsl_ctx* c, const char* what);
@@ -405,7 +394,9 @@ int fused_stage(fsl_ctx* ctx, int stage)
     a.dtY = v.dtY;
     a.dtZ = v.dtZ;
     ctx_prof(ctx, K_FUSED_XYZ, 0);
-    switch (fused_variant()) {
+    if (v.precision == FSL_PREC_MIXED) {
+        launch_fused_mixed(v, stage, in, out);
+    } else switch (fused_variant()) {
     case 1: launch_variant<16, 16, 1>(v.stream, stage, a, 1); break;
     case 2: launch_variant<16, 32, 2>(v.stream, stage, a, 1); break;
     case 3: launch_variant<16, 32, 1>(v.stream, stage, a, 1); break;

@@ -1,0 +1,330 @@
+// fsl_fused_mixed.hip -- mixed-precision fused stage kernel.
+//
+// The decomposition of fsl_fused.hip (16 x 16 workgroup tile with a one-cell
+// halo ring marching along x; x faces in registers, y/z faces and viscous
+// data through LDS) with the FP32 arithmetic of fsl_device_mixed.hpp.  The
+// state, the face states, pressure recovery and the RK accumulation stay in
+// FP64; fluxes, 1/rho-velocities and viscous cell data travel through LDS in
+// FP32, which halves the kernel's register and LDS footprint.
+#include <stdlib.h>
+
+#include "fsl.h"
+#include "fsl_device_mixed.hpp"
+#include "fsl_internal.hpp"
+
+namespace fsl {
+
+
+namespace {
+
+struct MixedArgs {
+    CState5 in;
+    CState5 base;
+    State5 out;
+    const double* mu;
+    Geo g;
+    Phys ph;
+    Bnd b;
+    double dtX, dtY, dtZ;
+    int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+};
+
+__device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
+{
+#pragma unroll
+    for (int f = 0; f < 5; f++) q[f] = s.f[f][c];
+}
+
+__device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
+
+template <int STAGE, int BJ, int BK, int MINW>
+__global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
+{
+    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
+    __shared__ float s_vr[3][3][NT];    // 1/rho velocities, planes i-1, i, i+1
+    __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
+    __shared__ double s_ym[5][NT];      // - face states along y
+    __shared__ double s_zm[5][NT];      // - face states along z
+    __shared__ float s_fy[5][NT];       // y flux through the + face
+    __shared__ float s_fz[5][NT];       // z flux through the + face
+
+    const Geo& g = a.g;
+    const int L = g.L, H = g.H, D = g.D;
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    if (bid >= a.nblocks) return;
+    const int tk = bid % a.ntk;
+    const int rem = bid / a.ntk;
+    const int tj = rem % a.ntj;
+    const int seg = rem / a.ntj;
+    const int k0 = 1 + tk * TK, j0 = 1 + tj * TJ;
+    const int i0 = 1 + seg * a.segL;
+    const int i1 = min(i0 + a.segL, L + 1);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int c = ty * BK + tx;
+    const int j = j0 - 1 + ty, k = k0 - 1 + tx;
+    const int jc = min(j, H + 1), kc = min(k, D + 1);
+    const bool interior = tx >= 1 && tx <= TK && ty >= 1 && ty <= TJ && j <= H && k <= D;
+    const bool yface = tx >= 1 && tx <= TK && ty <= TJ && k <= D && j <= H;
+    const bool zface = ty >= 1 && ty <= TJ && tx <= TK && j <= H && k <= D;
+    const int jm = max(jc - 1, 0), jp = min(jc + 1, H + 1);
+    int zkm = kc - 1, zkp = kc + 1;
+    if (kc == 0) { zkm = D - 1; zkp = D + 1; }
+    else if (kc == D + 1) { zkm = 0; zkp = 2; }
+    const long long sj = g.sj, si = g.si;
+    const long long col = (long long)jc * sj + kc;
+    const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
+    const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
+
+    double w0[5], w1[5], w2[5];
+    double xp_prev[5];
+    float Fx_prev[5], ly[5], lz[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) { xp_prev[f] = 0.; Fx_prev[f] = 0.f; ly[f] = 0.f; lz[f] = 0.f; }
+    {
+        const int ia = max(i0 - 2, 0), ib = i0 - 1;
+        load5(a.in, (long long)ia * si + col, w1);
+        load5(a.in, (long long)ib * si + col, w2);
+        const float ri = mx::rcp((float)w2[0]);
+        const int r = ring(ib);
+        s_vr[r][0][c] = (float)w2[1] * ri;
+        s_vr[r][1][c] = (float)w2[2] * ri;
+        s_vr[r][2][c] = (float)w2[3] * ri;
+    }
+
+    for (int i = i0 - 1; i <= i1; i++) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
+        load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
+        __syncthreads();
+        const int vc = i & 1, vb = vc ^ 1;
+        {
+            const float ri = mx::rcp((float)w2[0]);
+            const int r = ring(i + 1);
+            s_vr[r][0][c] = (float)w2[1] * ri;
+            s_vr[r][1][c] = (float)w2[2] * ri;
+            s_vr[r][2][c] = (float)w2[3] * ri;
+            float vd[3], Tc;
+            mx::cell_visc(w1, vd, Tc);
+            s_visc[vc][0][c] = vd[0];
+            s_visc[vc][1][c] = vd[1];
+            s_visc[vc][2][c] = vd[2];
+            s_visc[vc][3][c] = Tc;
+            s_visc[vc][4][c] = (float)w1[0];
+        }
+        mx::PrimF P;
+        mx::cell_prims(w1, P);
+        const bool inseg = (i >= i0 && i < i1);
+        double yp[5], zp[5];
+        if (inseg) {
+            const long long pl = (long long)i * si;
+            double qp[5], qm[5], mi5[5];
+            load5(a.in, pl + (long long)jp * sj + kc, qp);
+            load5(a.in, pl + (long long)jm * sj + kc, qm);
+            mx::recon_dir<2, 3, 1>(P, w1, qp, qm, false, yp, mi5);
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_ym[f][c] = mi5[f];
+            load5(a.in, pl + (long long)jc * sj + zkp, qp);
+            load5(a.in, pl + (long long)jc * sj + zkm, qm);
+            mx::recon_dir<3, 1, 2>(P, w1, qp, qm, a.ph.z_quirk != 0, zp, mi5);
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_zm[f][c] = mi5[f];
+        }
+        double xp[5], xm[5];
+        mx::recon_dir<1, 2, 3>(P, w1, w2, w0, false, xp, xm);
+        __syncthreads();
+
+        if (i >= i0 && interior) {
+            double l[5], r[5];
+            if (i == 1) {
+                inflow_face(a.b, H, j - 1, k - 1, l);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; f++) l[f] = xp_prev[f];
+            }
+            if (i == L + 1) {
+#pragma unroll
+                for (int f = 0; f < 5; f++) r[f] = xp_prev[f];
+                outflow_state(a.b.p_out, r);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; f++) r[f] = xm[f];
+            }
+            float R, vel[3], Pf;
+            mx::riemann<0>(l, r, R, vel, Pf);
+            const int rc = ring(i), rb = ring(i - 1);
+            const float vf[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
+            const float vbk[3] = {s_visc[vb][0][c], s_visc[vb][1][c], s_visc[vb][2][c]};
+            float gr[3][3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                gr[q][0] = (vf[q] - vbk[q]) * rd[0];
+                gr[q][1] = (s_vr[rc][q][c + BK] + s_vr[rb][q][c + BK] - s_vr[rc][q][c - BK] -
+                            s_vr[rb][q][c - BK]) * r4[1];
+                gr[q][2] = (s_vr[rc][q][c + 1] + s_vr[rb][q][c + 1] - s_vr[rc][q][c - 1] -
+                            s_vr[rb][q][c - 1]) * r4[2];
+            }
+            float mu_dyn = 0.f;
+            if (a.ph.dynamic) {
+                const long long m = (long long)i * si + col;
+                mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
+            }
+            float Fx[5];
+            mx::face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
+                             s_visc[vb][4][c], gr, mu_dyn, Fx);
+            if (i - 1 >= i0) {
+                const long long cc = (long long)(i - 1) * si + col;
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    const double lx = a.dtX * ((double)Fx_prev[f] - (double)Fx[f]);
+                    const double s = w0[f] + lx + (double)ly[f] + (double)lz[f];
+                    double o;
+                    if (STAGE == 1) o = s;
+                    else if (STAGE == 2) o = 0.75 * a.base.f[f][cc] + 0.25 * s;
+                    else o = (1. / 3.) * a.base.f[f][cc] + (2. / 3.) * s;
+                    a.out.f[f][cc] = o;
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
+        }
+#pragma unroll
+        for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
+
+        if (inseg) {
+            const int rp = ring(i + 1), rc = ring(i), rm = ring(i - 1);
+            const float vbk[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
+            const float Tb = s_visc[vc][3][c], Rb = s_visc[vc][4][c];
+            if (yface) {
+                double l[5], r[5];
+                if (j == 0) {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) l[f] = s_ym[f][c + BK];
+                    l[2] = -l[2];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) l[f] = yp[f];
+                }
+                if (j == H) {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) r[f] = yp[f];
+                    r[2] = -r[2];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 5; f++) r[f] = s_ym[f][c + BK];
+                }
+                float R, vel[3], Pf;
+                mx::riemann<1>(l, r, R, vel, Pf);
+                const float vf[3] = {s_visc[vc][0][c + BK], s_visc[vc][1][c + BK], s_visc[vc][2][c + BK]};
+                float gr[3][3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    gr[q][1] = (vf[q] - vbk[q]) * rd[1];
+                    gr[q][0] = (s_vr[rp][q][c + BK] + s_vr[rp][q][c] - s_vr[rm][q][c + BK] - s_vr[rm][q][c]) *
+                               r4[0];
+                    gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
+                                s_vr[rc][q][c - 1]) * r4[2];
+                }
+                float mu_dyn = 0.f;
+                if (a.ph.dynamic) {
+                    const long long m = (long long)i * si + (long long)(jc + 1) * sj + kc;
+                    mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
+                }
+                float Fy[5];
+                mx::face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + BK], Tb, s_visc[vc][4][c + BK], Rb,
+                                 gr, mu_dyn, Fy);
+#pragma unroll
+                for (int f = 0; f < 5; f++) s_fy[f][c] = Fy[f];
+            }
+            if (zface) {
+                double l[5], r[5];
+#pragma unroll
+                for (int f = 0; f < 5; f++) { l[f] = zp[f]; r[f] = s_zm[f][c + 1]; }
+                float R, vel[3], Pf;
+                mx::riemann<2>(l, r, R, vel, Pf);
+                const float vf[3] = {s_visc[vc][0][c + 1], s_visc[vc][1][c + 1], s_visc[vc][2][c + 1]};
+                float gr[3][3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    gr[q][2] = (vf[q] - vbk[q]) * rd[2];
+                    gr[q][0] = (s_vr[rp][q][c + 1] + s_vr[rp][q][c] - s_vr[rm][q][c + 1] - s_vr[rm][q][c]) * r4[0];
+                    gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
+                                s_vr[rc][q][c - BK]) * r4[1];
+                }
+                float mu_dyn = 0.f;
+                if (a.ph.dynamic) {
+                    const long long m = (long long)i * si + (long long)jc * sj + kc + 1;
+                    mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
+                }
+                float Fz[5];
+                mx::face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + 1], Tb, s_visc[vc][4][c + 1], Rb, gr,
+                                 mu_dyn, Fz);
+#pragma unroll
+                for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
+            }
+        }
+        __syncthreads();
+        if (inseg && interior) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                ly[f] = (float)(a.dtY * ((double)s_fy[f][c - BK] - (double)s_fy[f][c]));
+                lz[f] = (float)(a.dtZ * ((double)s_fz[f][c - 1] - (double)s_fz[f][c]));
+            }
+        }
+    }
+}
+
+template <int BJ, int BK, int MINW>
+void launch_mixed(hipStream_t s, int stage, MixedArgs a, int per_cu)
+{
+    const int TJ = BJ - 2, TK = BK - 2;
+    a.ntj = (a.g.H + TJ - 1) / TJ;
+    a.ntk = (a.g.D + TK - 1) / TK;
+    const int cols = a.ntj * a.ntk;
+    const int target = 4 * 256 * per_cu;
+    int nseg = (target + cols - 1) / cols;
+    if (nseg > a.g.L) nseg = a.g.L;
+    if (nseg < 1) nseg = 1;
+    int segL = (a.g.L + nseg - 1) / nseg;
+    if (segL < 4 && a.g.L >= 4) segL = 4;
+    nseg = (a.g.L + segL - 1) / segL;
+    a.segL = segL;
+    a.nseg = nseg;
+    a.nblocks = nseg * cols;
+    a.xcd_remap = 1;
+    const dim3 grid((a.nblocks + 7) / 8 * 8), block(BK, BJ);
+    if (stage == 1) hipLaunchKernelGGL((k_fused_mixed<1, BJ, BK, MINW>), grid, block, 0, s, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_fused_mixed<2, BJ, BK, MINW>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused_mixed<3, BJ, BK, MINW>), grid, block, 0, s, a);
+}
+
+}  // namespace
+
+void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* out)
+{
+    MixedArgs a{};
+    for (int f = 0; f < 5; f++) {
+        a.in.f[f] = in + f * v.g.G;
+        a.base.f[f] = v.U + f * v.g.G;
+        a.out.f[f] = out + f * v.g.G;
+    }
+    a.mu = v.mu;
+    a.g = v.g;
+    a.ph = v.ph;
+    a.b = v.bnd;
+    a.dtX = v.dtX;
+    a.dtY = v.dtY;
+    a.dtZ = v.dtZ;
+    static int minw = -1;
+    if (minw < 0) {
+        const char* e = getenv("FSL_MIXED_MINW");
+        minw = (e && atoi(e) == 3) ? 3 : 2;
+    }
+    if (minw == 3) launch_mixed<16, 16, 3>(v.stream, stage, a, 3);
+    else launch_mixed<16, 16, 2>(v.stream, stage, a, 2);
+}
+
+}  // namespace fsl

@@ -76,4 +76,19 @@ void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Ge
 void launch_filter(hipStream_t s, const double* in, double* out, double* t1, double* t2, const Geo& g,
                    const double h[3], void (*prof)(void*, int, int), void* prof_arg);
 
+// Snapshot of a context's launch parameters for the fused-path translation
+// units (fsl_api.hip builds it).
+struct CtxView {
+    hipStream_t stream;
+    Geo g;
+    Phys ph;
+    Bnd bnd;
+    double *U, *Up, *Uq, *mu;
+    double dtX, dtY, dtZ;
+    int cur_is_p;
+    int sgs_model;
+    int precision;
+};
+void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* out);
+
 }  // namespace fsl
