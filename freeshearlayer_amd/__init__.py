@@ -23,6 +23,7 @@ FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
 FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 
 class FslCase(C.Structure):
@@ -51,6 +52,23 @@ class FslBlock(C.Structure):
                 ("Ud", C.c_double), ("Vd", C.c_double), ("Wd", C.c_double)]
 
 
+class FslShearStats(C.Structure):
+    _fields_ = [("delta_theta", C.c_double), ("delta_theta_out", C.c_double), ("delta_w", C.c_double),
+                ("tke", C.c_double)]
+
+
+class FslLearnedStatus(C.Structure):
+    _fields_ = [("interval", C.c_int), ("tol", C.c_double), ("in_fallback", C.c_int),
+                ("last_error", C.c_double), ("judged", C.c_longlong), ("fallbacks", C.c_longlong),
+                ("learned_steps", C.c_longlong), ("dynamic_steps", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MLP_NPARAM = 4801
+
+
 class FslShearCase(C.Structure):
     _fields_ = [("mach_c", C.c_double), ("ratio", C.c_double), ("delta_w0", C.c_double),
                 ("reynolds", C.c_double), ("pert", C.c_double), ("rho0", C.c_double),
@@ -70,6 +88,7 @@ def load():
                            "`python -m freeshearlayer_amd.build` (no fallback exists)" % LIB_PATH)
     L = C.CDLL(LIB_PATH)
     vp, i_, d_, u_, cc = C.c_void_p, C.c_int, C.c_double, C.c_uint, C.c_char_p
+    ll = C.c_longlong
     sig = {
         "fsl_case_read_ini": (i_, [cc, C.POINTER(FslCase)]),
         "fsl_case_read_bin": (i_, [cc, C.POINTER(FslCase)]),
@@ -131,6 +150,33 @@ def load():
         "fsl_kernel_name": (cc, [i_]),
         "fsl_kernel_count": (i_, []),
         "fsl_copy_kernel": (i_, [vp, C.c_size_t, i_]),
+        "fsl_plane_profiles": (i_, [vp, _dp]),
+        "fsl_shear_statistics": (i_, [C.POINTER(FslCase), _dp, d_, d_, d_, _dp, _dp,
+                                      C.POINTER(FslShearStats)]),
+        "fsl_mlp_init_params": (None, [C.c_ulonglong, _fp, _fp]),
+        "fsl_mlp_file_write": (i_, [cc, _fp, _fp]),
+        "fsl_mlp_file_read": (i_, [cc, _fp, _fp]),
+        "fsl_mlp_set_params": (i_, [vp, _fp, _fp]),
+        "fsl_mlp_get_params": (i_, [vp, _fp, _fp]),
+        "fsl_mlp_forward": (i_, [vp, _fp, ll, _fp]),
+        "fsl_mlp_features": (i_, [vp, ll, ll, ll, _fp]),
+        "fsl_mlp_update_thickness": (i_, [vp]),
+        "fsl_mlp_get_thickness": (i_, [vp, _fp]),
+        "fsl_mlp_set_thickness": (i_, [vp, _fp]),
+        "fsl_learned_smagorinsky": (i_, [vp]),
+        "fsl_mlp_reserve": (i_, [vp, ll]),
+        "fsl_mlp_collect": (i_, [vp, ll, ll, C.POINTER(ll)]),
+        "fsl_mlp_sample_count": (ll, [vp]),
+        "fsl_mlp_clear_samples": (i_, [vp]),
+        "fsl_mlp_samples": (i_, [vp, ll, ll, _fp, _fp]),
+        "fsl_mlp_load_samples": (i_, [vp, _fp, _fp, ll]),
+        "fsl_mlp_fit_norm": (i_, [vp, _fp]),
+        "fsl_mlp_train": (i_, [vp, i_, ll, C.c_float, C.c_ulonglong, _fp]),
+        "fsl_mlp_gradient": (i_, [vp, ll, _fp, _fp]),
+        "fsl_learned_configure": (i_, [vp, i_, d_]),
+        "fsl_learned_get_status": (i_, [vp, C.POINTER(FslLearnedStatus)]),
+        "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
+        "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -152,6 +198,17 @@ def _ptr(a):
     if a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a):
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_fp)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(shape) if shape is not None else a
 
 
 def case_reference(L, H, D, **over):
@@ -332,6 +389,157 @@ class Context:
     @property
     def device_bytes(self):
         return self.lib.fsl_device_bytes(self.h)
+
+    # --- shear-layer statistics ---
+    def plane_profiles(self):
+        c = self.case
+        out = np.zeros((7, c.LEN, c.HIG))
+        self._chk(self.lib.fsl_plane_profiles(self.h, _ptr(out)), "plane_profiles")
+        return out
+
+    # --- learned closure ---
+    def mlp_set_params(self, params, norm):
+        p, n = _f32(params), _f32(norm)
+        if p.size != MLP_NPARAM or n.size != 16:
+            raise ValueError("expected %d parameters and 16 normalisation values" % MLP_NPARAM)
+        self._chk(self.lib.fsl_mlp_set_params(self.h, _fptr(p), _fptr(n)), "mlp_set_params")
+
+    def mlp_get_params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        n = np.zeros(16, np.float32)
+        self._chk(self.lib.fsl_mlp_get_params(self.h, _fptr(p), _fptr(n)), "mlp_get_params")
+        return p, n
+
+    def mlp_forward(self, X):
+        X = _f32(X)
+        n = X.shape[0]
+        y = np.zeros(n, np.float32)
+        self._chk(self.lib.fsl_mlp_forward(self.h, _fptr(X.reshape(-1)), n, _fptr(y)), "mlp_forward")
+        return y
+
+    def mlp_features(self, stride=1, offset=0, n=None):
+        N = self.case.LEN * self.case.HIG * self.case.DEP
+        if n is None:
+            n = (N - offset + stride - 1) // stride
+        X = np.zeros((n, 8), np.float32)
+        self._chk(self.lib.fsl_mlp_features(self.h, stride, offset, n, _fptr(X.reshape(-1))), "mlp_features")
+        return X
+
+    def mlp_update_thickness(self):
+        self._chk(self.lib.fsl_mlp_update_thickness(self.h), "mlp_update_thickness")
+
+    def mlp_thickness(self):
+        dw = np.zeros(self.case.LEN + 2, np.float32)
+        self._chk(self.lib.fsl_mlp_get_thickness(self.h, _fptr(dw)), "mlp_get_thickness")
+        return dw
+
+    def mlp_set_thickness(self, dw):
+        dw = _f32(dw)
+        self._chk(self.lib.fsl_mlp_set_thickness(self.h, _fptr(dw)), "mlp_set_thickness")
+
+    def learned_smagorinsky(self):
+        self._chk(self.lib.fsl_learned_smagorinsky(self.h), "learned_smagorinsky")
+
+    def mlp_reserve(self, cap):
+        self._chk(self.lib.fsl_mlp_reserve(self.h, cap), "mlp_reserve")
+
+    def mlp_collect(self, stride, offset=0):
+        n = C.c_longlong(0)
+        self._chk(self.lib.fsl_mlp_collect(self.h, stride, offset, C.byref(n)), "mlp_collect")
+        return n.value
+
+    def mlp_sample_count(self):
+        return self.lib.fsl_mlp_sample_count(self.h)
+
+    def mlp_clear_samples(self):
+        self._chk(self.lib.fsl_mlp_clear_samples(self.h), "mlp_clear_samples")
+
+    def mlp_samples(self, first=0, n=None):
+        if n is None:
+            n = self.mlp_sample_count() - first
+        X = np.zeros((n, 8), np.float32)
+        t = np.zeros(n, np.float32)
+        self._chk(self.lib.fsl_mlp_samples(self.h, first, n, _fptr(X.reshape(-1)), _fptr(t)), "mlp_samples")
+        return X, t
+
+    def mlp_load_samples(self, X, t):
+        X = _f32(X)
+        t = _f32(t)
+        self._chk(self.lib.fsl_mlp_load_samples(self.h, _fptr(X.reshape(-1)), _fptr(t), t.size),
+                  "mlp_load_samples")
+
+    def mlp_fit_norm(self):
+        n = np.zeros(16, np.float32)
+        self._chk(self.lib.fsl_mlp_fit_norm(self.h, _fptr(n)), "mlp_fit_norm")
+        return n
+
+    def mlp_train(self, epochs, batch, lr, seed=0):
+        loss = np.zeros(max(epochs, 1), np.float32)
+        self._chk(self.lib.fsl_mlp_train(self.h, epochs, batch, lr, seed, _fptr(loss)), "mlp_train")
+        return loss[:epochs]
+
+    def mlp_gradient(self, batch):
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self.lib.fsl_mlp_gradient(self.h, batch, _fptr(g), _fptr(loss)), "mlp_gradient")
+        return g, float(loss[0])
+
+    def learned_configure(self, interval=50, tol=0.10):
+        self._chk(self.lib.fsl_learned_configure(self.h, interval, tol), "learned_configure")
+
+    def learned_status(self):
+        s = FslLearnedStatus()
+        self._chk(self.lib.fsl_learned_get_status(self.h, C.byref(s)), "learned_status")
+        return s.as_dict()
+
+    def learned_judge(self):
+        e, dd, dl = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.lib.fsl_learned_judge(self.h, C.byref(e), C.byref(dd), C.byref(dl)), "learned_judge")
+        return e.value, dd.value, dl.value
+
+    def mfma_probe(self, A, B):
+        A = _f32(A, (16, 16))
+        B = _f32(B, (16, 16))
+        D = np.zeros((16, 16), np.float32)
+        self._chk(self.lib.fsl_mfma_probe(self.h, _fptr(A), _fptr(B), _fptr(D)), "mfma_probe")
+        return D
+
+
+def shear_statistics(case, prof, U1, U2, rho0=1.0):
+    """delta_theta(x), delta_w(x) and integrated resolved TKE from plane profiles."""
+    prof = np.ascontiguousarray(prof, dtype=np.float64)
+    th = np.zeros(case.LEN)
+    dw = np.zeros(case.LEN)
+    st = FslShearStats()
+    rc = load().fsl_shear_statistics(C.byref(case), _ptr(prof.reshape(-1)), U1, U2, rho0, _ptr(th), _ptr(dw),
+                                     C.byref(st))
+    if rc != 0:
+        raise FslError("fsl_shear_statistics returned %d" % rc)
+    return {"delta_theta": st.delta_theta, "delta_theta_out": st.delta_theta_out, "delta_w": st.delta_w,
+            "tke": st.tke, "delta_theta_x": th, "delta_w_x": dw}
+
+
+def mlp_init_params(seed=0):
+    p = np.zeros(MLP_NPARAM, np.float32)
+    n = np.zeros(16, np.float32)
+    load().fsl_mlp_init_params(seed, _fptr(p), _fptr(n))
+    return p, n
+
+
+def mlp_file_write(path, params, norm):
+    p, n = _f32(params), _f32(norm)
+    rc = load().fsl_mlp_file_write(path.encode(), _fptr(p), _fptr(n))
+    if rc != 0:
+        raise FslError("fsl_mlp_file_write(%s) returned %d" % (path, rc))
+
+
+def mlp_file_read(path):
+    p = np.zeros(MLP_NPARAM, np.float32)
+    n = np.zeros(16, np.float32)
+    rc = load().fsl_mlp_file_read(path.encode(), _fptr(p), _fptr(n))
+    if rc != 0:
+        raise FslError("fsl_mlp_file_read(%s) returned %d" % (path, rc))
+    return p, n
 
 
 def ic_reference(case):

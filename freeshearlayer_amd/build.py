@@ -38,8 +38,9 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-std=c99", "-Wall", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE]
 
-HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip"]
-HOST_SOURCES = ["fsl_host.c", "fsl_driver.c"]
+HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip",
+               "fsl_mlp.hip", "fsl_learned.hip"]
+HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c"]
 
 
 def _run(cmd, cwd=None):

@@ -14,8 +14,7 @@
 #include <new>
 #include <vector>
 
-#include "fsl.h"
-#include "fsl_internal.hpp"
+#include "fsl_ctx.hpp"
 
 using namespace fsl;
 
@@ -24,55 +23,23 @@ namespace fsl {
 int fused_stage(fsl_ctx* ctx, int stage);
 }
 
-struct ProfRec {
-    int id;
-    hipEvent_t a, b;
-};
+namespace fsl {
 
-struct fsl_ctx {
-    fsl_case cs;
-    fsl_config cfg;
-    Geo g;
-    Phys ph;
-    Bnd bnd;
-    size_t nface[3];
-    hipStream_t stream;
-
-    double* U;          // 5 G
-    double* Up;         // 5 G
-    double* Uq;         // 5 G, fused path only
-    double* faces;      // 10 (NX + NY + NZ), unfused path only
-    double* mu;         // G
-    double* sgs;        // scratch for the dynamic model
-    double* filt;       // scratch of fsl_filter (in, out and two pass buffers)
-    double* u_in;       // HIG
-    double* sums;       // partials + 5 results
-    unsigned long long* red;
-    size_t bytes;
-
-    int cur_is_p;       // stage input: U (0) or Up (1)
-    double dt, dtX, dtY, dtZ;
-    fsl_block blk;
-    unsigned step;
-    int sgs_valid_step; // step whose mu_SGS is current (fused path), -1 none
-
-    int prof;
-    int prof_open_id;
-    hipEvent_t prof_open_ev;
-    std::vector<ProfRec> pending;
-    std::vector<hipEvent_t> pool;
-    double prof_ms[K_NKERNELS];
-    long long prof_cnt[K_NKERNELS];
-
-    char err[512];
-};
-
-namespace {
+Bnd bnd_now(fsl_ctx* c)
+{
+    Bnd b = c->bnd;
+    b.k_min = c->blk.k_min;
+    b.k_max = c->blk.k_max;
+    b.Ud = c->blk.Ud;
+    b.Vd = c->blk.Vd;
+    b.Wd = c->blk.Wd;
+    return b;
+}
 
 const char* kNames[K_NKERNELS] = {
     "ghost_fill", "recon", "iface_bc", "flux_x", "flux_y", "flux_z", "rk_update", "cfl_reduce",
     "sgs_filter_a", "filter_y", "filter_z", "filter_x", "sgs_strain", "sgs_filter_b", "fused_x",
-    "fused_y", "fused_z", "sums", "copy", "fused_stage"};
+    "fused_y", "fused_z", "sums", "copy", "fused_stage", "mlp_mu", "mlp_train", "plane_profiles"};
 
 int set_err(fsl_ctx* c, int code, const char* fmt, ...)
 {
@@ -84,13 +51,6 @@ int set_err(fsl_ctx* c, int code, const char* fmt, ...)
     }
     return code;
 }
-
-#define HIPCHK(ctx, call)                                                                        \
-    do {                                                                                         \
-        hipError_t e_ = (call);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return set_err((ctx), FSL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 hipEvent_t pool_get(fsl_ctx* c)
 {
@@ -234,7 +194,7 @@ void derive(fsl_ctx* c)
     ph.z_quirk = c->cfg.z_quirk;
 }
 
-}  // namespace
+}  // namespace fsl
 
 // ----------------------------------------------------------------------
 extern "C" {
@@ -268,7 +228,8 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
     c->cs = *cs;
     if (cfg) c->cfg = *cfg;
     else fsl_config_defaults(&c->cfg);
-    if (c->cfg.precision != FSL_PREC_FP64 && c->cfg.precision != FSL_PREC_MIXED) {
+    if ((c->cfg.precision != FSL_PREC_FP64 && c->cfg.precision != FSL_PREC_MIXED) ||
+        c->cfg.sgs_model < FSL_SGS_CONSTANT || c->cfg.sgs_model > FSL_SGS_LEARNED) {
         delete c;
         return FSL_ERR_ARG;
     }
@@ -324,6 +285,8 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         // reference inflow: 76.4 m/s for rows j <= HIG/2, 200 above
         std::vector<double> u(H);
         for (int j = 1; j <= H; j++) u[j - 1] = (j <= H / 2) ? 76.4 : 200;
+        c->dU = 200 - 76.4;
+        c->dw0 = 2 * cs->deltaY;   // one-cell jump over a two-cell central difference
         if (hipMemcpyAsync(c->u_in, u.data(), H * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
             hipSuccess) {
             rc = set_err(c, FSL_ERR_HIP, "inflow upload");
@@ -363,6 +326,7 @@ void fsl_destroy(fsl_ctx* c)
     (void)hipFree(c->u_in);
     (void)hipFree(c->sums);
     (void)hipFree(c->red);
+    learned_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -386,6 +350,16 @@ int fsl_get_config(const fsl_ctx* c, fsl_config* out)
 
 int fsl_set_inflow(fsl_ctx* c, const double* u_in)
 {
+    double lo = u_in[0], hi = u_in[0];
+    for (int j = 1; j < c->g.H; j++) {
+        lo = fmin(lo, u_in[j]);
+        hi = fmax(hi, u_in[j]);
+    }
+    c->dU = hi - lo;
+    double gmax = 0;
+    for (int j = 1; j + 1 < c->g.H; j++) gmax = fmax(gmax, fabs(u_in[j + 1] - u_in[j - 1]) / (2 * c->cs.deltaY));
+    c->dw0 = gmax > 0 ? c->dU / gmax : c->g.H * c->cs.deltaY;
+    c->lrn.dw_valid = 0;
     HIPCHK(c, hipMemcpyAsync(c->u_in, u_in, c->g.H * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSL_OK;
@@ -532,17 +506,6 @@ int fsl_synchronize(fsl_ctx* c)
 
 // --- operations ------------------------------------------------------
 
-static Bnd bnd_now(fsl_ctx* c)
-{
-    Bnd b = c->bnd;
-    b.k_min = c->blk.k_min;
-    b.k_max = c->blk.k_max;
-    b.Ud = c->blk.Ud;
-    b.Vd = c->blk.Vd;
-    b.Wd = c->blk.Wd;
-    return b;
-}
-
 int fsl_ghost_fill(fsl_ctx* c)
 {
     prof_mark(c, K_GHOST, 0);
@@ -575,7 +538,7 @@ int fsl_dynamic_smagorinsky(fsl_ctx* c)
 {
     int rc = ensure_sgs(c);
     if (rc) return rc;
-    launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c);
+    launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c, nullptr);
     return check_launch(c, "dynamic_smagorinsky");
 }
 
@@ -586,6 +549,10 @@ int fsl_fluxes(fsl_ctx* c)
     if (c->cfg.sgs_model == FSL_SGS_DYNAMIC) {
         // src/fluxes.c:52-54: the model runs on the step state U1..U4
         rc = fsl_dynamic_smagorinsky(c);
+        if (rc) return rc;
+    } else if (c->cfg.sgs_model == FSL_SGS_LEARNED && !c->cur_is_p) {
+        // once per step, at the stage that reads the step state
+        rc = learned_sgs_step(c);
         if (rc) return rc;
     }
     const Faces F = faces_of(c);

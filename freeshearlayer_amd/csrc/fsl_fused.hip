@@ -30,6 +30,7 @@ CtxView ctx_view(fsl_ctx* c);
 void ctx_prof(fsl_ctx* c, int id, int end);
 int ctx_check(fsl_ctx* c, const char* what);
 int ctx_sgs(fsl_ctx* c);
+int learned_sgs_step(fsl_ctx* c);
 int ctx_ghost(fsl_ctx* c, const double* state);
 void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
@@ -379,7 +380,11 @@ int fused_stage(fsl_ctx* ctx, int stage)
         // the model reads the step state only: once per step (src/fluxes.c:53)
         rc = ctx_sgs(ctx);
         if (rc) return rc;
+    } else if (stage == 1 && v.sgs_model == FSL_SGS_LEARNED) {
+        rc = learned_sgs_step(ctx);
+        if (rc) return rc;
     }
+    v = ctx_view(ctx);   // the learned closure may have swapped the mu_SGS array
     FusedArgs a{};
     for (int f = 0; f < 5; f++) {
         a.in.f[f] = in + f * v.g.G;

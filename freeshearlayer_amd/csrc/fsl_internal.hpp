@@ -34,6 +34,9 @@ enum KernelId {
     K_SUM,
     K_COPY,
     K_FUSED_XYZ,
+    K_MLP_MU,
+    K_MLP_TRAIN,
+    K_DIAG,
     K_NKERNELS
 };
 
@@ -71,7 +74,31 @@ struct SgsBufs {
 };
 size_t sgs_scratch_arrays();   // number of cell arrays SgsBufs needs
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
-                void (*prof)(void*, int, int), void* prof_arg);
+                void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
+
+// --- learned closure (fsl_mlp.hip) ---
+int mlp_nparam();
+int mlp_npack();   // packed BF16 operand entries (8 bytes each)
+void launch_mlp_pack(hipStream_t s, const float* master, void* pack);
+void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* pack, const float* master,
+                        const float* norm, float* y);
+void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
+                   double dy, const void* pack, const float* master, const float* norm, double* mu,
+                   float* cd_out);
+void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
+                         double dy, const float* cd, long long stride, long long offset, float* X, float* t,
+                         long long nout);
+void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
+                           long long step, unsigned long long pa, unsigned long long pc, const void* pack,
+                           const float* master, const float* norm, float* grad, float* loss);
+void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t);
+void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D);
+int dissipation_nblocks();
+void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu,
+                            const float* dw, double* partial);
+void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out);   // 7 L H doubles
+void launch_diag_dw(hipStream_t s, const double* zs, const Geo& g, double dU, double dy, float fallback,
+                    float* dw);
 // one box filter of a cell array (y, z, x passes)
 void launch_filter(hipStream_t s, const double* in, double* out, double* t1, double* t2, const Geo& g,
                    const double h[3], void (*prof)(void*, int, int), void* prof_arg);

@@ -1,0 +1,707 @@
+// fsl_mlp.hip -- learned SGS closure: a per-cell MLP (8 -> 64 -> 64 -> 1,
+// tanh) that predicts the dynamic-Smagorinsky coefficient Cd = Cs^2 from
+// normalised invariants of the resolved strain and rotation, |S| Delta and
+// the distance from the layer centre, replacing the test filter and the
+// Germano-Lilly least squares of src/turbulence.c:103-507.  mu_SGS is then
+// rho Cd Delta^2 |S| as in src/turbulence.c:442.
+//
+// All dense contractions run on the matrix cores: v_mfma_f32_16x16x16_bf16
+// (BF16 operands, FP32 accumulation).  One wavefront pushes 64 cells through
+// the network as 4 x 4 tiles of 16 x 16; activations move between layers
+// through LDS in BF16.  Training uses the same tiles for the forward pass,
+// the transposed products of the backward pass and the weight gradients,
+// which are reduced with FP32 atomics and applied by an Adam step.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "fsl_internal.hpp"
+
+namespace fsl {
+
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int NIN = 8, NH = 64;
+constexpr int KP1 = 16;   // layer-1 K padded to the MFMA depth
+
+// parameter offsets in the FP32 master vector
+constexpr int OW1 = 0, OB1 = OW1 + NIN * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
+              OB3 = OW3 + NH, NPARAM = OB3 + 1;
+static_assert(NPARAM == 4801, "parameter count");
+
+// packed BF16 B operands, lane order (16x16x16 MFMA: lane l holds
+// B[k = 4 (l / 16) + e][n = l % 16], e = 0..3)
+constexpr int PB1 = 0;                  // W1: 4 n-tiles
+constexpr int PB2 = PB1 + 4 * 64;       // W2: 4 k-steps x 4 n-tiles
+constexpr int PB2T = PB2 + 16 * 64;     // W2 transposed (backward): 4 k-steps x 4 n-tiles
+constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
+constexpr int NPACK = PB3 + 4 * 64;
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+__device__ __forceinline__ float tanh_fast(float x)
+{
+    // tanh x = 1 - 2 / (exp(2x) + 1), exp through v_exp_f32 (2^x); BF16
+    // activations keep 8 bits, far coarser than this evaluation's error
+    const float xc = fminf(fmaxf(x, -15.f), 15.f);
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * xc);   // 2 log2(e) x
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
+struct MlpW {
+    const bf4* pack;      // NPACK / 64 operand tiles, bf4 per lane
+    const float* master;  // biases (and w3 for the backward pass)
+    const float* norm;    // mean[8], 1/std[8]
+};
+
+// LDS workspace of one wavefront
+struct WaveLds {
+    unsigned short X[64 * KP1];
+    unsigned short H1[64 * NH];
+    unsigned short H2[64 * NH];
+    float y[64];
+};
+
+__device__ __forceinline__ bf4 lds_row4(const unsigned short* p)
+{
+    const unsigned short* q = p;
+    bf4 v;
+    v.x = (short)q[0]; v.y = (short)q[1]; v.z = (short)q[2]; v.w = (short)q[3];
+    return v;
+}
+
+// Forward pass of 64 samples held in ws.X (BF16, already normalised);
+// leaves H1, H2 (BF16) in LDS and the outputs in ws.y.  Caller syncs.
+__device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
+{
+    const int r16 = lane & 15, q4 = lane >> 4;
+    // layer 1: X (64 x 16) W1 (16 x 64)
+    for (int tm = 0; tm < 4; tm++) {
+        const bf4 a = lds_row4(&ws.X[(tm * 16 + r16) * KP1 + 4 * q4]);
+        for (int tn = 0; tn < 4; tn++) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(a, w.pack[(PB1 >> 6) * 64 + tn * 64 + lane], acc);
+            const float b = w.master[OB1 + tn * 16 + r16];
+#pragma unroll
+            for (int e = 0; e < 4; e++) ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(tanh_fast(acc[e] + b));
+        }
+    }
+    __syncthreads();
+    // layer 2: H1 (64 x 64) W2 (64 x 64)
+    for (int tm = 0; tm < 4; tm++) {
+        f4 acc[4];
+        for (int tn = 0; tn < 4; tn++) acc[tn] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 4; ks++) {
+            const bf4 a = lds_row4(&ws.H1[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
+            for (int tn = 0; tn < 4; tn++) acc[tn] = mfma(a, w.pack[PB2 / 64 * 64 + (ks * 4 + tn) * 64 + lane], acc[tn]);
+        }
+        for (int tn = 0; tn < 4; tn++) {
+            const float b = w.master[OB2 + tn * 16 + r16];
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(tanh_fast(acc[tn][e] + b));
+        }
+    }
+    __syncthreads();
+    // layer 3: H2 (64 x 64) w3 (64 x 16, column 0)
+    for (int tm = 0; tm < 4; tm++) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 4; ks++) {
+            const bf4 a = lds_row4(&ws.H2[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
+            acc = mfma(a, w.pack[PB3 / 64 * 64 + ks * 64 + lane], acc);
+        }
+        if (r16 == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) ws.y[tm * 16 + 4 * q4 + e] = acc[e] + w.master[OB3];
+        }
+    }
+    __syncthreads();
+}
+
+// ---- features ----
+struct FeatArgs {
+    CState5 U;
+    Geo g;
+    Phys ph;
+    double dU;        // velocity difference of the two streams
+    const float* dw;  // vorticity thickness delta_w(x) per x index (L + 2)
+    double dy;
+};
+
+// Velocity-gradient features of interior cell (i, j, k); returns |S| =
+// sqrt(2 S:S) (the model's strain magnitude, src/turbulence.c:299-301).
+__device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, int k, float f[NIN])
+{
+    const Geo& g = a.g;
+    const long long c = cidx(g, i, j, k);
+    const long long st[3] = {g.si, g.sj, 1};
+    float gr[3][3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const long long p = c + st[e], m = c - st[e];
+        const float rp = __builtin_amdgcn_rcpf((float)a.U.f[0][p]);
+        const float rm = __builtin_amdgcn_rcpf((float)a.U.f[0][m]);
+        const float h = (float)a.ph.r2del[e];
+#pragma unroll
+        for (int r = 0; r < 3; r++) gr[r][e] = ((float)a.U.f[1 + r][p] * rp - (float)a.U.f[1 + r][m] * rm) * h;
+    }
+    float S[3][3], W[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            S[r][e] = 0.5f * (gr[r][e] + gr[e][r]);
+            W[r][e] = 0.5f * (gr[r][e] - gr[e][r]);
+        }
+    float s2 = 0.f, w2 = 0.f, s3 = 0.f, sww = 0.f, ssww = 0.f;
+    float SS[3][3], WW[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            s2 += S[r][e] * S[r][e];
+            w2 += W[r][e] * W[r][e];
+            float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                a1 += S[r][q] * S[q][e];
+                a2 += W[r][q] * W[q][e];
+            }
+            SS[r][e] = a1;
+            WW[r][e] = a2;
+        }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            s3 += SS[r][e] * S[e][r];
+            sww += S[r][e] * WW[e][r];
+            ssww += SS[r][e] * WW[e][r];
+        }
+    const float tiny = 1e-30f;
+    const float smag = sqrtf(2.f * s2);
+    const float sn = sqrtf(s2);
+    const float om[3] = {gr[2][1] - gr[1][2], gr[0][2] - gr[2][0], gr[1][0] - gr[0][1]};
+    float ws = 0.f, oo = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        oo += om[r] * om[r];
+#pragma unroll
+        for (int e = 0; e < 3; e++) ws += om[r] * S[r][e] * om[e];
+    }
+    const float yv = ((float)j - (float)(g.H / 2) - 0.5f) * (float)a.dy;
+    f[0] = (w2 - s2) / (w2 + s2 + tiny);
+    f[1] = s3 / (s2 * sn + tiny);
+    f[2] = sww / (sn * w2 + tiny);
+    f[3] = ssww / (s2 * w2 + tiny);
+    f[4] = (gr[0][0] + gr[1][1] + gr[2][2]) / (smag + tiny);
+    f[5] = ws / (oo * smag + tiny);
+    f[6] = smag * sqrtf((float)a.ph.DD) / (float)a.dU;
+    f[7] = yv * __builtin_amdgcn_rcpf(a.dw[i]);
+    return smag;
+}
+
+__device__ __forceinline__ void decode_interior(const Geo& g, long long t, int& i, int& j, int& k)
+{
+    k = (int)(t % g.D) + 1;
+    j = (int)((t / g.D) % g.H) + 1;
+    i = (int)(t / ((long long)g.D * g.H)) + 1;
+}
+
+// features (raw) and training target Cd / 0.15 of every `stride`-th interior cell
+__global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* cd, long long stride,
+                                                      long long offset, float* X, float* t, long long nout)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nout) return;
+    const long long cell = offset + s * stride;
+    int i, j, k;
+    decode_interior(a.g, cell, i, j, k);
+    float f[NIN];
+    cell_features(a, i, j, k, f);
+#pragma unroll
+    for (int q = 0; q < NIN; q++) X[s * NIN + q] = f[q];
+    if (t) t[s] = cd[cidx(a.g, i, j, k)] / 0.15f;
+}
+
+// learned mu_SGS for all interior cells (one wavefront = 64 cells)
+__global__ void __launch_bounds__(256) k_mlp_mu(FeatArgs a, MlpW w, double* mu, float* cd_out)
+{
+    __shared__ WaveLds lds[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds& ws = lds[wv];
+    const long long N = (long long)a.g.L * a.g.H * a.g.D;
+    const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
+    float f[NIN], smag = 0.f;
+    int i = 1, j = 1, k = 1;
+    const bool valid = cell < N;
+    if (valid) {
+        decode_interior(a.g, cell, i, j, k);
+        smag = cell_features(a, i, j, k, f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NIN; q++) f[q] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = f2bf((f[q] - w.norm[q]) * w.norm[NIN + q]);
+#pragma unroll
+    for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
+    __syncthreads();
+    forward64(ws, w, lane);
+    if (valid) {
+        const float Cd = fminf(fmaxf(0.15f * ws.y[lane], 0.f), 0.15f);
+        const long long c = cidx(a.g, i, j, k);
+        mu[c] = a.U.f[0][c] * (double)Cd * a.ph.DD * (double)smag;
+        if (cd_out) cd_out[c] = Cd;
+    }
+}
+
+// forward pass on caller-given raw features (normalised here as in k_mlp_mu)
+__global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, MlpW w, float* y)
+{
+    __shared__ WaveLds lds[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds& ws = lds[wv];
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = s < n ? f2bf((X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q]) : 0;
+#pragma unroll
+    for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
+    __syncthreads();
+    forward64(ws, w, lane);
+    if (s < n) y[s] = ws.y[lane];
+}
+
+// ---- training ----
+struct TrainArgs {
+    const float* X;      // raw features
+    const float* t;      // targets
+    long long n;         // samples
+    long long batch;     // samples per optimizer step (multiple of 256)
+    long long step;      // step index inside the epoch
+    unsigned long long perm_a, perm_c;   // sample permutation: (a * idx + c) mod n
+    float* grad;         // NPARAM accumulators
+    float* loss;         // sum of squared errors
+};
+
+__global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
+{
+    __shared__ WaveLds lds[4];
+    __shared__ unsigned short sdz[4][64 * NH];
+    __shared__ float sdy[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    WaveLds& ws = lds[wv];
+    unsigned short* dz = sdz[wv];
+    const long long idx = T.step * T.batch + (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long s = (long long)((T.perm_a * (unsigned long long)(idx % T.n) + T.perm_c) % (unsigned long long)T.n);
+#pragma unroll
+    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = f2bf((T.X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q]);
+#pragma unroll
+    for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
+    __syncthreads();
+    forward64(ws, w, lane);
+    const float err = ws.y[lane] - T.t[s];
+    const float dy = err * (2.f / (float)T.batch);
+    sdy[wv][lane] = dy;
+    atomicAdd(T.loss, err * err);
+    __syncthreads();
+    // layer 3: grad w3[k] = sum_s H2[s][k] dy[s], grad b3; dZ2 = dy w3 (1 - H2^2)
+    {
+        float gw = 0.f, gb = 0.f;
+        for (int q = 0; q < 64; q++) {
+            gw += bf2f(ws.H2[q * NH + lane]) * sdy[wv][q];
+            gb += sdy[wv][q];
+        }
+        atomicAdd(&T.grad[OW3 + lane], gw);
+        if (lane == 0) atomicAdd(&T.grad[OB3], gb);
+        for (int kk = 0; kk < NH; kk++) {
+            const float h = bf2f(ws.H2[lane * NH + kk]);
+            dz[lane * NH + kk] = f2bf(dy * w.master[OW3 + kk] * (1.f - h * h));
+        }
+    }
+    __syncthreads();
+    // grad W2 = H1^T dZ2 (M = j, N = k, K = samples); grad b2 = column sums of dZ2
+    {
+        float gb = 0.f;
+        for (int q = 0; q < 64; q++) gb += bf2f(dz[q * NH + lane]);
+        atomicAdd(&T.grad[OB2 + lane], gb);
+        for (int tj = 0; tj < 4; tj++)
+            for (int tk = 0; tk < 4; tk++) {
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int ks = 0; ks < 4; ks++) {
+                    bf4 a, b;
+                    const int s0 = ks * 16 + 4 * q4;
+                    a.x = (short)ws.H1[(s0 + 0) * NH + tj * 16 + r16];
+                    a.y = (short)ws.H1[(s0 + 1) * NH + tj * 16 + r16];
+                    a.z = (short)ws.H1[(s0 + 2) * NH + tj * 16 + r16];
+                    a.w = (short)ws.H1[(s0 + 3) * NH + tj * 16 + r16];
+                    b.x = (short)dz[(s0 + 0) * NH + tk * 16 + r16];
+                    b.y = (short)dz[(s0 + 1) * NH + tk * 16 + r16];
+                    b.z = (short)dz[(s0 + 2) * NH + tk * 16 + r16];
+                    b.w = (short)dz[(s0 + 3) * NH + tk * 16 + r16];
+                    acc = mfma(a, b, acc);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    atomicAdd(&T.grad[OW2 + (tj * 16 + 4 * q4 + e) * NH + tk * 16 + r16], acc[e]);
+            }
+    }
+    // dZ1 = (dZ2 W2^T) (1 - H1^2) -> H2 buffer
+    {
+        f4 acc[4][4];
+        for (int tm = 0; tm < 4; tm++)
+            for (int tn = 0; tn < 4; tn++) acc[tm][tn] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int tm = 0; tm < 4; tm++)
+            for (int ks = 0; ks < 4; ks++) {
+                const bf4 a = lds_row4(&dz[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
+                for (int tn = 0; tn < 4; tn++) acc[tm][tn] = mfma(a, w.pack[PB2T / 64 * 64 + (ks * 4 + tn) * 64 + lane], acc[tm][tn]);
+            }
+        __syncthreads();
+        for (int tm = 0; tm < 4; tm++)
+            for (int tn = 0; tn < 4; tn++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int sr = tm * 16 + 4 * q4 + e, jc = tn * 16 + r16;
+                    const float h = bf2f(ws.H1[sr * NH + jc]);
+                    ws.H2[sr * NH + jc] = f2bf(acc[tm][tn][e] * (1.f - h * h));
+                }
+    }
+    __syncthreads();
+    // grad W1 = X^T dZ1 (M = features padded to 16, N = j, K = samples); grad b1
+    {
+        float gb = 0.f;
+        for (int q = 0; q < 64; q++) gb += bf2f(ws.H2[q * NH + lane]);
+        atomicAdd(&T.grad[OB1 + lane], gb);
+        for (int tn = 0; tn < 4; tn++) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 4; ks++) {
+                bf4 a, b;
+                const int s0 = ks * 16 + 4 * q4;
+                a.x = (short)ws.X[(s0 + 0) * KP1 + r16];
+                a.y = (short)ws.X[(s0 + 1) * KP1 + r16];
+                a.z = (short)ws.X[(s0 + 2) * KP1 + r16];
+                a.w = (short)ws.X[(s0 + 3) * KP1 + r16];
+                b.x = (short)ws.H2[(s0 + 0) * NH + tn * 16 + r16];
+                b.y = (short)ws.H2[(s0 + 1) * NH + tn * 16 + r16];
+                b.z = (short)ws.H2[(s0 + 2) * NH + tn * 16 + r16];
+                b.w = (short)ws.H2[(s0 + 3) * NH + tn * 16 + r16];
+                acc = mfma(a, b, acc);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int fr = 4 * q4 + e;
+                if (fr < NIN) atomicAdd(&T.grad[OW1 + fr * NH + tn * 16 + r16], acc[e]);
+            }
+        }
+    }
+}
+
+struct AdamArgs {
+    float* master;
+    float* grad;
+    float* m;
+    float* v;
+    float lr, b1, b2, eps;
+    float c1, c2;   // 1 / (1 - b1^t), 1 / (1 - b2^t)
+};
+
+__global__ void k_mlp_adam(AdamArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    const float g = a.grad[p];
+    const float m = a.b1 * a.m[p] + (1.f - a.b1) * g;
+    const float v = a.b2 * a.v[p] + (1.f - a.b2) * g * g;
+    a.m[p] = m;
+    a.v[p] = v;
+    a.master[p] -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+    a.grad[p] = 0.f;
+}
+
+// BF16 B operands in lane order from the FP32 master weights
+__global__ void k_mlp_pack(const float* master, bf4* pack)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // one (tile, lane) per thread
+    if (t >= NPACK) return;
+    const int tile = t / 64, lane = t % 64;
+    const int r16 = lane & 15, q4 = lane >> 4;
+    float v[4];
+    for (int e = 0; e < 4; e++) {
+        const int kk = 4 * q4 + e;
+        float x = 0.f;
+        if (tile < PB2 / 64) {                      // W1, n-tile = tile
+            const int n = tile * 16 + r16;
+            x = kk < NIN ? master[OW1 + kk * NH + n] : 0.f;
+        } else if (tile < PB2T / 64) {              // W2 [k][n]
+            const int q = tile - PB2 / 64, ks = q / 4, tn = q % 4;
+            x = master[OW2 + (ks * 16 + kk) * NH + tn * 16 + r16];
+        } else if (tile < PB3 / 64) {               // W2^T: B[k][n] = W2[n][k]
+            const int q = tile - PB2T / 64, ks = q / 4, tn = q % 4;
+            x = master[OW2 + (tn * 16 + r16) * NH + ks * 16 + kk];
+        } else {                                    // w3 in column 0
+            const int ks = tile - PB3 / 64;
+            x = r16 == 0 ? master[OW3 + ks * 16 + kk] : 0.f;
+        }
+        v[e] = x;
+    }
+    bf4 o;
+    o.x = (short)f2bf(v[0]);
+    o.y = (short)f2bf(v[1]);
+    o.z = (short)f2bf(v[2]);
+    o.w = (short)f2bf(v[3]);
+    pack[t] = o;
+}
+
+// one 16x16x16 product with the assumed operand layouts (layout check)
+__global__ void k_mfma_probe(const float* A, const float* B, float* D)
+{
+    const int lane = threadIdx.x, r16 = lane & 15, q4 = lane >> 4;
+    bf4 a, b;
+    a.x = (short)f2bf(A[r16 * 16 + 4 * q4 + 0]);
+    a.y = (short)f2bf(A[r16 * 16 + 4 * q4 + 1]);
+    a.z = (short)f2bf(A[r16 * 16 + 4 * q4 + 2]);
+    a.w = (short)f2bf(A[r16 * 16 + 4 * q4 + 3]);
+    b.x = (short)f2bf(B[(4 * q4 + 0) * 16 + r16]);
+    b.y = (short)f2bf(B[(4 * q4 + 1) * 16 + r16]);
+    b.z = (short)f2bf(B[(4 * q4 + 2) * 16 + r16]);
+    b.w = (short)f2bf(B[(4 * q4 + 3) * 16 + r16]);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(a, b, acc);
+#pragma unroll
+    for (int e = 0; e < 4; e++) D[(4 * q4 + e) * 16 + r16] = acc[e];
+}
+
+struct SumArgs {
+    CState5 U;
+    Geo g;
+    Phys ph;
+    const double* mu;
+    const float* dw;
+};
+
+// sum over interior cells of mu |S|^2 (SGS dissipation proxy), per block
+__global__ void __launch_bounds__(256) k_sgs_dissipation(SumArgs a, double* partial)
+{
+    __shared__ double red[256];
+    const long long N = (long long)a.g.L * a.g.H * a.g.D;
+    double s = 0.;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_interior(a.g, t, i, j, k);
+        FeatArgs fa;
+        fa.U = a.U;
+        fa.g = a.g;
+        fa.ph = a.ph;
+        fa.dU = 1.;
+        fa.dw = a.dw;
+        fa.dy = 1.;
+        float f[NIN];
+        const float sm = cell_features(fa, i, j, k, f);
+        s += a.mu[cidx(a.g, i, j, k)] * (double)sm * (double)sm;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+
+// z sums of rho, rho u, rho v, rho w, (rho u)^2 / rho, (rho v)^2 / rho,
+// (rho w)^2 / rho for every interior (i, j): one wavefront per (i, j) row,
+// fixed lane-strided order and a fixed butterfly, so the sums are
+// reproducible.  Layout out[q][(i - 1) H + (j - 1)], q = 0..6.
+__global__ void __launch_bounds__(256) k_diag_zsum(CState5 U, Geo g, double* out)
+{
+    const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const long long nrow = (long long)g.L * g.H;
+    if (row >= nrow) return;
+    const int i = (int)(row / g.H) + 1, j = (int)(row % g.H) + 1;
+    const long long base = cidx(g, i, j, 0);
+    double s[7] = {0., 0., 0., 0., 0., 0., 0.};
+    for (int k = 1 + lane; k <= g.D; k += 64) {
+        const double r = U.f[0][base + k], mu = U.f[1][base + k], mv = U.f[2][base + k], mw = U.f[3][base + k];
+        const double ir = 1. / r;
+        s[0] += r;
+        s[1] += mu;
+        s[2] += mv;
+        s[3] += mw;
+        s[4] += mu * mu * ir;
+        s[5] += mv * mv * ir;
+        s[6] += mw * mw * ir;
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        double v = s[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) out[q * nrow + row] = v;
+    }
+}
+
+// delta_w(x) = dU / max_j |d u~ / dy| from the Favre-averaged profile
+// u~ = sum(rho u) / sum(rho) of each x plane (one block per i); planes whose
+// profile is flat keep the fallback value.
+__global__ void __launch_bounds__(256) k_diag_dw(const double* zs, Geo g, double dU, double dy, float fallback,
+                                                 float* dw)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x + 1;
+    const long long nrow = (long long)g.L * g.H;
+    const double* sr = zs + (long long)(i - 1) * g.H;
+    const double* su = zs + nrow + (long long)(i - 1) * g.H;
+    double m = 0.;
+    for (int j = 1 + threadIdx.x; j < g.H - 1; j += 256) {
+        const double up = su[j + 1] / sr[j + 1], um = su[j - 1] / sr[j - 1];
+        m = fmax(m, fabs(up - um) / (2. * dy));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double d = red[0] > 0. ? dU / red[0] : (double)fallback;
+        dw[i] = (float)d;
+        if (i == 1) dw[0] = (float)d;
+        if (i == g.L) dw[g.L + 1] = (float)d;
+    }
+}
+
+}  // namespace
+
+// ---- launch layer ----
+int mlp_nparam() { return NPARAM; }
+int mlp_npack() { return NPACK; }
+
+void launch_mlp_pack(hipStream_t s, const float* master, void* pack)
+{
+    hipLaunchKernelGGL(k_mlp_pack, dim3((NPACK + 255) / 256), dim3(256), 0, s, master, (bf4*)pack);
+}
+
+static MlpW mlpw(const void* pack, const float* master, const float* norm)
+{
+    MlpW w;
+    w.pack = (const bf4*)pack;
+    w.master = master;
+    w.norm = norm;
+    return w;
+}
+
+void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* pack, const float* master,
+                        const float* norm, float* y)
+{
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, n,
+                       mlpw(pack, master, norm), y);
+}
+
+void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
+                   double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out)
+{
+    FeatArgs a;
+    a.U = U;
+    a.g = g;
+    a.ph = ph;
+    a.dU = dU;
+    a.dw = dw;
+    a.dy = dy;
+    const long long N = (long long)g.L * g.H * g.D;
+    hipLaunchKernelGGL(k_mlp_mu, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a,
+                       mlpw(pack, master, norm), mu, cd_out);
+}
+
+void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
+                         double dy, const float* cd, long long stride, long long offset, float* X, float* t,
+                         long long nout)
+{
+    FeatArgs a;
+    a.U = U;
+    a.g = g;
+    a.ph = ph;
+    a.dU = dU;
+    a.dw = dw;
+    a.dy = dy;
+    hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, a, cd, stride,
+                       offset, X, t, nout);
+}
+
+void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
+                           long long step, unsigned long long pa, unsigned long long pc, const void* pack,
+                           const float* master, const float* norm, float* grad, float* loss)
+{
+    TrainArgs T;
+    T.X = X;
+    T.t = t;
+    T.n = n;
+    T.batch = batch;
+    T.step = step;
+    T.perm_a = pa;
+    T.perm_c = pc;
+    T.grad = grad;
+    T.loss = loss;
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)(batch / 256)), dim3(256), 0, s, T, mlpw(pack, master, norm));
+}
+
+void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t)
+{
+    AdamArgs a;
+    a.master = master;
+    a.grad = grad;
+    a.m = m;
+    a.v = v;
+    a.lr = lr;
+    a.b1 = 0.9f;
+    a.b2 = 0.999f;
+    a.eps = 1e-8f;
+    a.c1 = 1.f / (1.f - powf(a.b1, (float)t));
+    a.c2 = 1.f / (1.f - powf(a.b2, (float)t));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D)
+{
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D);
+}
+
+int dissipation_nblocks() { return 1024; }
+
+void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu,
+                            const float* dw, double* partial)
+{
+    SumArgs a;
+    a.U = U;
+    a.g = g;
+    a.ph = ph;
+    a.mu = mu;
+    a.dw = dw;
+    hipLaunchKernelGGL(k_sgs_dissipation, dim3(1024), dim3(256), 0, s, a, partial);
+}
+
+void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out)
+{
+    const long long nrow = (long long)g.L * g.H;
+    hipLaunchKernelGGL(k_diag_zsum, dim3((unsigned)((nrow + 3) / 4)), dim3(256), 0, s, U, g, out);
+}
+
+void launch_diag_dw(hipStream_t s, const double* zs, const Geo& g, double dU, double dy, float fallback, float* dw)
+{
+    hipLaunchKernelGGL(k_diag_dw, dim3((unsigned)g.L), dim3(256), 0, s, zs, g, dU, dy, fallback, dw);
+}
+
+}  // namespace fsl

@@ -314,6 +314,7 @@ struct SgsB {
     const double* Aij[6];
     const double* mag;
     double* mu;
+    float* cd;   // optional: the clipped coefficient (training targets)
 };
 
 __global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
@@ -384,6 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
         if (Cd > 0.15) Cd = 0.15;
         else if (Cd < 0.0) Cd = 0.0;
         a.mu[c] = a.rho[c] * Cd * ph.DD * a.mag[c];
+        if (a.cd) a.cd[c] = (float)Cd;
     }
 }
 
@@ -425,7 +427,7 @@ void launch_filter(hipStream_t s, const double* in, double* out, double* t1, dou
 }
 
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
-                void (*prof)(void*, int, int), void* pa)
+                void (*prof)(void*, int, int), void* pa, float* cd_out)
 {
     const int ntiles = ((g.L + TI - 1) / TI) * ((g.H + TJ - 1) / TJ) * ((g.D + TK - 1) / TK);
     SgsA A{};
@@ -448,6 +450,7 @@ void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Ge
     }
     Bk.mag = B.mag;
     Bk.mu = mu;
+    Bk.cd = cd_out;
     if (prof) prof(pa, K_SGS_FINAL, 0);
     hipLaunchKernelGGL(k_sgs_b, dim3(ntiles), dim3(kBlock), 0, s, Bk, g, ph);
     if (prof) prof(pa, K_SGS_FINAL, 1);

@@ -249,6 +249,104 @@ int fsl_synchronize(fsl_ctx *ctx);
 /* --- diagnostics (device FP64 reductions) --- */
 /* out[0..4]: sums of rho, rho u, rho v, rho w, E over the interior */
 int fsl_conservation_sums(fsl_ctx *ctx, double out[5]);
+/* Plane profiles of the step state: z sums of rho, rho u, rho v, rho w,
+   (rho u)^2/rho, (rho v)^2/rho, (rho w)^2/rho for every interior (i, j);
+   out[q LEN HIG + (i-1) HIG + (j-1)], q = 0..6 (7 LEN HIG doubles). */
+int fsl_plane_profiles(fsl_ctx *ctx, double *out);
+
+/* Shear-layer statistics from those profiles (host).  With the Favre mean
+   u~ = sum(rho u) / sum(rho) and rho_bar = sum(rho) / DEP of each (i, j):
+     delta_theta(x_i) = 1 / (rho0 dU^2) sum_j rho_bar (U1 - u~)(u~ - U2) deltaY
+     delta_w(x_i)     = dU / max_j |u~(j+1) - u~(j-1)| / (2 deltaY)
+     tke              = sum_{i,j} 1/2 sum_c [(rho u_c)^2/rho sums - (sum rho u_c)^2 / sum rho] * cell volume
+   with dU = U1 - U2 (U1 the faster stream).  Averages of delta_theta and
+   delta_w are over the planes i > LEN / 2.  dtheta_x (LEN values) and dw_x
+   may be NULL. */
+typedef struct fsl_shear_stats {
+    double delta_theta;      /* mean over i > LEN/2 */
+    double delta_theta_out;  /* at i = LEN */
+    double delta_w;          /* mean over i > LEN/2 */
+    double tke;              /* resolved (Favre) TKE integrated over the domain, J */
+} fsl_shear_stats;
+int fsl_shear_statistics(const fsl_case *c, const double *prof, double U1, double U2, double rho0,
+                         double *dtheta_x, double *dw_x, fsl_shear_stats *out);
+
+/* --- learned SGS closure (sgs_model = FSL_SGS_LEARNED) --- */
+/* Network 8 -> 64 -> 64 -> 1 with tanh hidden layers; BF16 operands on the
+   matrix cores with FP32 accumulation.  Output y gives Cd = clip(0.15 y, 0,
+   0.15) and mu_SGS = rho Cd Delta^2 |S| (src/turbulence.c:436-445 with the
+   network's Cd in place of the Germano-Lilly one).  Features per interior
+   cell, from central differences of u = (rho u)/rho (FP32):
+     0 (|W|^2 - |S|^2) / (|W|^2 + |S|^2)    4 div u / |S|
+     1 tr(S^3) / |S|_F^3                     5 omega.S.omega / (|omega|^2 |S|)
+     2 tr(S W^2) / (|S|_F |W|_F^2)          6 |S| Delta / dU
+     3 tr(S^2 W^2) / (|S|_F^2 |W|_F^2)      7 y / delta_w(x)
+   with |S| = sqrt(2 S:S), |S|_F^2 = S:S, dU the inflow velocity difference
+   and delta_w(x) the vorticity thickness of the x plane, refreshed at every
+   judge.  Parameters: FP32 vector W1[8][64], b1[64], W2[64][64], b2[64],
+   w3[64], b3 (FSL_MLP_NPARAM values); norm[16]: feature means, then
+   reciprocal standard deviations (the network sees (f - mean) * rstd). */
+#define FSL_MLP_NIN 8
+#define FSL_MLP_NHID 64
+#define FSL_MLP_NPARAM 4801
+/* host: Glorot-uniform weights (splitmix64 of seed), zero biases, identity norm */
+void fsl_mlp_init_params(unsigned long long seed, float *params, float *norm);
+/* parameter file: "FSLMLP01", int32 nin, nhid, nparam, float norm[16], float params[nparam] */
+int fsl_mlp_file_write(const char *path, const float *params, const float *norm);
+int fsl_mlp_file_read(const char *path, float *params, float *norm);
+int fsl_mlp_set_params(fsl_ctx *ctx, const float *params, const float *norm);
+int fsl_mlp_get_params(fsl_ctx *ctx, float *params, float *norm);
+/* network output for n host rows of 8 raw features (normalised with norm) */
+int fsl_mlp_forward(fsl_ctx *ctx, const float *X, long long n, float *y);
+/* raw features of interior cells offset, offset + stride, ... (n of them;
+   interior order k fastest, then j, then i) of the step state U */
+int fsl_mlp_features(fsl_ctx *ctx, long long stride, long long offset, long long n, float *X);
+/* delta_w(x) from U; read (LEN+2 floats) or set it */
+int fsl_mlp_update_thickness(fsl_ctx *ctx);
+int fsl_mlp_get_thickness(fsl_ctx *ctx, float *dw);
+int fsl_mlp_set_thickness(fsl_ctx *ctx, const float *dw);
+/* learned mu_SGS of the step state U (ghost layer filled by the caller) */
+int fsl_learned_smagorinsky(fsl_ctx *ctx);
+/* Training-sample store on the device.  fsl_mlp_collect fills the ghost
+   layer of U, runs the dynamic model and appends the features and targets
+   Cd / 0.15 of interior cells offset, offset + stride, ... up to the
+   reserved capacity (mu_SGS is left holding the dynamic model's values). */
+int fsl_mlp_reserve(fsl_ctx *ctx, long long cap);
+int fsl_mlp_collect(fsl_ctx *ctx, long long stride, long long offset, long long *n_total);
+long long fsl_mlp_sample_count(const fsl_ctx *ctx);
+int fsl_mlp_clear_samples(fsl_ctx *ctx);
+int fsl_mlp_samples(fsl_ctx *ctx, long long first, long long n, float *X, float *t);
+int fsl_mlp_load_samples(fsl_ctx *ctx, const float *X, const float *t, long long n);
+/* feature means and reciprocal standard deviations of the store -> norm[16] */
+int fsl_mlp_fit_norm(fsl_ctx *ctx, float *norm);
+/* Adam (0.9, 0.999, 1e-8) on the mean squared error of y against the
+   targets; epochs over a fresh permutation each, batch a multiple of 256;
+   epoch_loss (epochs values, may be NULL) receives each epoch's mean loss. */
+int fsl_mlp_train(fsl_ctx *ctx, int epochs, long long batch, float lr, unsigned long long seed,
+                  float *epoch_loss);
+/* gradient of the mean squared error over samples 0 .. batch-1 (batch a
+   multiple of 256) at the current parameters, without an update; loss
+   receives the sum of squared errors */
+int fsl_mlp_gradient(fsl_ctx *ctx, long long batch, float *grad, float *loss);
+/* Judge: every interval steps (steps 1, 1 + interval, ...) both models run
+   and the relative difference of the SGS dissipation sum(mu_SGS |S|^2) is
+   the error; above tol the dynamic model is used, and kept every step until
+   a later judge passes.  Defaults: interval 50, tol 0.10; interval 0 never
+   judges. */
+int fsl_learned_configure(fsl_ctx *ctx, int interval, double tol);
+typedef struct fsl_learned_status {
+    int interval;
+    double tol;
+    int in_fallback;
+    double last_error;       /* -1 before the first judge */
+    long long judged, fallbacks, learned_steps, dynamic_steps;
+} fsl_learned_status;
+int fsl_learned_get_status(const fsl_ctx *ctx, fsl_learned_status *s);
+/* one judge evaluation on U now (no state change beyond mu_SGS = dynamic) */
+int fsl_learned_judge(fsl_ctx *ctx, double *err, double *d_dynamic, double *d_learned);
+/* one v_mfma_f32_16x16x16_bf16 product D = A B of row-major 16x16 FP32
+   matrices rounded to BF16 (operand layout check) */
+int fsl_mfma_probe(fsl_ctx *ctx, const float *A, const float *B, float *D);
 
 /* --- measurement --- */
 /* When on, every kernel launch is bracketed by HIP events on the context's

@@ -1,0 +1,271 @@
+"""Learned SGS closure on the device: the MFMA operand layout, the BF16
+network (forward and gradient) against a numpy restatement with the same
+rounding steps, the per-cell features and delta_w(x) against numpy, the
+learned mu_SGS, training, and the judge / fallback logic of the step.
+
+Tolerances: the MFMA product of BF16-exact operands differs from FP64 only
+by FP32 accumulation (rtol 1e-5).  Network outputs differ from the numpy
+restatement where the device tanh (exp2 / rcp) and np.tanh round a hidden
+activation to different BF16 neighbours (one BF16 step, 2^-8 relative, in
+a few activations): max |dy| <= 2e-2.  Features are FP32 on the device and
+FP64 in numpy: errors within 2e-4 of each feature's scale, and within 1e-4
+relative for 99 % of cells.  The fallback path must reproduce the dynamic model bit for bit."""
+import numpy as np
+import pytest
+
+import freeshearlayer_amd as fsl
+from fsl_test_util import random_state
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def shear_ctx(L=32, H=24, D=16, sgs=fsl.FSL_SGS_LEARNED, fused=1, precision=fsl.FSL_PREC_FP64, seed=7):
+    case = fsl.case_reference(L, H, D, BL_HIG=min(5, H - H // 2))
+    sh = fsl.shear_defaults()
+    sh.seed = seed
+    U, u_in, mu = fsl.ic_shear_layer(case, sh)
+    case.mu_L = mu
+    cfg = fsl.config(sgs_model=sgs, fused=fused, precision=precision, cfl_mode=fsl.FSL_CFL_ACOUSTIC, cfl=0.5)
+    ctx = fsl.Context(case, cfg)
+    ctx.set_inflow(u_in)
+    ctx.upload_state(U)
+    ctx.set_lcg(0.317391239817, 0)
+    return ctx, case, U, u_in
+
+
+def test_mfma_operand_layout():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((16, 16)).astype(np.float32)
+    B = rng.standard_normal((16, 16)).astype(np.float32)
+    ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        D = ctx.mfma_probe(A, B)
+    ref = mlp_ref.bf16(A).astype(np.float64) @ mlp_ref.bf16(B).astype(np.float64)
+    np.testing.assert_allclose(D, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_forward_matches_numpy():
+    rng = np.random.default_rng(2)
+    p, _ = fsl.mlp_init_params(11)
+    p[mlp_ref.OB1:mlp_ref.OW2] = rng.uniform(-0.3, 0.3, 64)
+    p[mlp_ref.OB2:mlp_ref.OW3] = rng.uniform(-0.3, 0.3, 64)
+    norm = np.concatenate([rng.uniform(-1, 1, 8), rng.uniform(0.5, 2, 8)]).astype(np.float32)
+    X = rng.standard_normal((1000, 8)).astype(np.float32) * 1.5   # not a multiple of 256
+    ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        ctx.mlp_set_params(p, norm)
+        y = ctx.mlp_forward(X)
+        q, m = ctx.mlp_get_params()
+    assert np.array_equal(q, p) and np.array_equal(m, norm)
+    yr, _ = mlp_ref.forward(X, p, norm)
+    ye, _ = mlp_ref.forward(X, p, norm, exact=True)
+    d = np.abs(y - yr)
+    print("forward: max |y - y_bf16| %.2e, mean %.2e, max |y - y_fp32| %.2e"
+          % (d.max(), d.mean(), np.abs(y - ye).max()))
+    assert d.max() < 2e-2 and d.mean() < 2e-3
+    assert np.abs(y - ye).max() < 0.1 * max(1.0, np.abs(ye).max())
+
+
+def test_gradient_matches_numpy():
+    rng = np.random.default_rng(3)
+    p, norm = fsl.mlp_init_params(5)
+    X = rng.standard_normal((1024, 8)).astype(np.float32)
+    t = (0.5 + 0.3 * np.tanh(X[:, 0]) - 0.2 * X[:, 1] * X[:, 2]).astype(np.float32)
+    ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        ctx.mlp_set_params(p, norm)
+        ctx.mlp_load_samples(X, t)
+        g, loss = ctx.mlp_gradient(1024)
+    gr, lr = mlp_ref.gradient(X, t, p, norm)
+    assert abs(loss - lr) <= 2e-2 * lr
+    groups = {"W1": (0, 512), "b1": (512, 576), "W2": (576, 4672), "b2": (4672, 4736), "w3": (4736, 4800),
+              "b3": (4800, 4801)}
+    for name, (a, b) in groups.items():
+        ga, gb = g[a:b].astype(np.float64), gr[a:b]
+        scale = np.abs(gb).max()
+        err = np.abs(ga - gb).max() / scale
+        cos = ga @ gb / (np.linalg.norm(ga) * np.linalg.norm(gb))
+        print("grad %s: max rel %.2e, cos %.6f" % (name, err, cos))
+        assert err < 3e-2, name
+        assert cos > 0.999, name
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((65536, 8)).astype(np.float32) * 2 + 1
+    t = (0.5 + 0.3 * np.tanh(X[:, 0]) - 0.05 * X[:, 1] * X[:, 2]).astype(np.float32)
+    ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        p, norm = fsl.mlp_init_params(9)
+        ctx.mlp_load_samples(X, t)
+        norm = ctx.mlp_fit_norm()
+        np.testing.assert_allclose(norm[:8], X.mean(0), atol=2e-3)
+        np.testing.assert_allclose(norm[8:], 1 / X.std(0), rtol=2e-3)
+        ctx.mlp_set_params(p, norm)
+        loss = ctx.mlp_train(8, 1024, 3e-3, seed=1)
+        q, _ = ctx.mlp_get_params()
+        y = ctx.mlp_forward(X[:4096])
+    print("epoch losses:", ["%.3e" % v for v in loss])
+    assert np.all(np.isfinite(loss))
+    assert loss[-1] < 0.2 * loss[0]
+    # the final epoch's mean loss describes the trained network
+    assert np.mean((y - t[:4096]) ** 2) < 2 * loss[-1] + 1e-4
+    assert not np.array_equal(q, p)
+
+
+def test_plane_profiles_and_thickness():
+    ctx, case, U, u_in = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        for _ in range(3):
+            ctx.step()
+        S = ctx.download_state()
+        prof = ctx.plane_profiles()
+        ctx.mlp_update_thickness()
+        dw = ctx.mlp_thickness()
+    pr = mlp_ref.plane_profiles(S)
+    for q in range(7):   # sums in another order: FP64 rounding of the field's scale
+        np.testing.assert_allclose(prof[q], pr[q], rtol=1e-12, atol=1e-13 * np.abs(pr[q]).max())
+    dU = u_in.max() - u_in.min()
+    ref = mlp_ref.thickness(pr, dU, case.deltaY, 1.0)
+    np.testing.assert_allclose(dw, ref, rtol=1e-6)
+
+
+def test_features_and_learned_mu():
+    L, H, D = 20, 16, 12
+    case = fsl.case_reference(L, H, D, BL_HIG=5)
+    U = random_state(L, H, D, seed=21)
+    cfg = fsl.config(sgs_model=fsl.FSL_SGS_LEARNED, fused=1)
+    rng = np.random.default_rng(6)
+    with fsl.Context(case, cfg) as ctx:
+        u_in = np.linspace(80.0, 190.0, H)
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        dw = rng.uniform(2.0, 6.0, L + 2).astype(np.float32)
+        ctx.mlp_set_thickness(dw)
+        Xg = ctx.mlp_features()
+        Xs = ctx.mlp_features(stride=7, offset=3)
+        p, _ = fsl.mlp_init_params(13)
+        norm = np.concatenate([Xg.mean(0), 1 / Xg.std(0)]).astype(np.float32)
+        ctx.mlp_set_params(p, norm)
+        ctx.learned_smagorinsky()
+        mu = ctx.download_mu()
+        y = ctx.mlp_forward(Xg)
+    f, smag = mlp_ref.features(U, case, u_in.max() - u_in.min(), dw)
+    # FP32 invariants cancel where |W| ~ |S| (feature 0) or where a
+    # denominator is small (2, 5): bound the error by each feature's scale,
+    # and the relative error of nearly all cells
+    scale = np.abs(f).max(0)
+    aerr = np.abs(Xg - f) / scale
+    rel = np.abs(Xg - f) / (np.abs(f) + 1e-6 * scale)
+    print("feature error / scale per column:", ["%.1e" % v for v in aerr.max(0)])
+    print("feature p99 rel error per column:", ["%.1e" % v for v in np.percentile(rel, 99, axis=0)])
+    assert aerr.max() < 2e-4
+    assert np.percentile(rel, 99, axis=0).max() < 1e-4
+    assert np.array_equal(Xs, Xg[3::7])
+    Cd = np.clip(0.15 * y.astype(np.float64), 0.0, 0.15)
+    DD = (case.deltaX * case.deltaY * case.deltaZ) ** 0.666666666666667
+    rho = U[0, 1:-1, 1:-1, 1:-1].reshape(-1)
+    mref = (rho * Cd * DD * smag).reshape(L, H, D)
+    np.testing.assert_allclose(mu[1:-1, 1:-1, 1:-1], mref, rtol=2e-5, atol=1e-12 * np.abs(mref).max())
+    # ghost layer stays zero
+    inner = np.zeros_like(mu, dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    assert np.all(mu[~inner] == 0)
+    assert (Cd > 0).any() and (Cd < 0.15).any()
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_fallback_reproduces_dynamic_bitwise(fused):
+    """A network that predicts Cd = 0 fails every judge (error 1): each step
+    then runs on the dynamic model's mu_SGS, so the state matches the
+    dynamic-model run bit for bit."""
+    a, case, U, u_in = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_LEARNED, fused=fused)
+    b, *_ = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_DYNAMIC, fused=fused)
+    p = np.zeros(fsl.MLP_NPARAM, np.float32)
+    p[-1] = -1.0
+    norm = np.concatenate([np.zeros(8), np.ones(8)]).astype(np.float32)
+    with a, b:
+        a.mlp_set_params(p, norm)
+        a.learned_configure(interval=1, tol=0.10)
+        for _ in range(3):
+            a.step()
+            b.step()
+        st = a.learned_status()
+        assert np.array_equal(a.download_state(), b.download_state())
+    print(st)
+    assert st["judged"] == 3 and st["fallbacks"] == 3 and st["dynamic_steps"] == 3
+    assert abs(st["last_error"] - 1.0) < 1e-12
+
+
+def test_judge_interval_and_learned_steps():
+    ctx, case, U, u_in = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_LEARNED, fused=1)
+    p, _ = fsl.mlp_init_params(17)
+    with ctx:
+        ctx.ghost_fill()   # features read the ghost layer of the step state
+        X = ctx.mlp_features()
+        norm = np.concatenate([X.mean(0), 1 / (X.std(0) + 1e-12)]).astype(np.float32)
+        ctx.mlp_set_params(p, norm)
+        ctx.learned_configure(interval=2, tol=1e30)   # judge steps 1, 3, 5; never fall back
+        for _ in range(5):
+            ctx.step()
+        st = ctx.learned_status()
+        S = ctx.download_state()
+        err, dd, dl = ctx.learned_judge()
+    print(st, err, dd, dl)
+    assert st["judged"] == 3 and st["fallbacks"] == 0 and st["learned_steps"] == 5
+    assert np.all(np.isfinite(S))
+    assert dd > 0 and dl > 0 and abs(err - abs(dl - dd) / dd) < 1e-12
+
+
+def test_learned_step_uses_network_mu():
+    """Between judges the step's mu_SGS is the network's on the step state."""
+    a, case, U, u_in = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_LEARNED, fused=1)
+    b, *_ = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_LEARNED, fused=1)
+    p, _ = fsl.mlp_init_params(19)
+    with a, b:
+        a.ghost_fill()
+        X = a.mlp_features()
+        norm = np.concatenate([X.mean(0), 1 / (X.std(0) + 1e-12)]).astype(np.float32)
+        for c in (a, b):
+            c.mlp_set_params(p, norm)
+            c.learned_configure(interval=0, tol=0.1)
+        a.step()
+        b.ghost_fill()
+        b.learned_smagorinsky()
+        assert np.array_equal(a.download_mu(), b.download_mu())
+
+
+def test_collect_and_train_on_solver_data():
+    ctx, case, U, u_in = shear_ctx(32, 24, 16, sgs=fsl.FSL_SGS_DYNAMIC, fused=1)
+    with ctx:
+        ctx.mlp_reserve(200000)
+        n = 0
+        for s in range(12):
+            ctx.step()
+            n = ctx.mlp_collect(stride=2, offset=s % 2)
+        N = 32 * 24 * 16
+        assert n == 12 * (N // 2)
+        X, t = ctx.mlp_samples()
+        assert np.all(np.isfinite(X)) and np.all((t >= 0) & (t <= 1))
+        # the last collection's targets are the dynamic model's Cd / 0.15 at those cells
+        S = ctx.download_state()
+        ctx.upload_state(S)
+        ctx.ghost_fill()
+        ctx.dynamic_smagorinsky()
+        mu = ctx.download_mu()[1:-1, 1:-1, 1:-1].reshape(-1)
+        f, smag = mlp_ref.features(S, case, u_in.max() - u_in.min(), np.ones(case.LEN + 2))
+        rho = S[0, 1:-1, 1:-1, 1:-1].reshape(-1)
+        DD = (case.deltaX * case.deltaY * case.deltaZ) ** 0.666666666666667
+        cd_back = mu / (rho * DD * np.maximum(smag, 1e-300))
+        last = t[-(N // 2):]
+        sel = smag[1::2] > 1e-3 * smag.max()
+        np.testing.assert_allclose(0.15 * last[sel], cd_back[1::2][sel], rtol=1e-3, atol=1e-6)
+        p, _ = fsl.mlp_init_params(23)
+        norm = ctx.mlp_fit_norm()
+        ctx.mlp_set_params(p, norm)
+        loss = ctx.mlp_train(6, 2048, 2e-3, seed=3)
+        err, dd, dl = ctx.learned_judge()
+    print("losses", loss, "judge error %.3f" % err)
+    assert np.all(np.isfinite(loss)) and loss[-1] < loss[0]
+    assert np.isfinite(err)
