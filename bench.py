@@ -1,24 +1,29 @@
 """Throughput benchmark of the per-step update on one MI355X.
 
-Workload (default): the GPU FP64 plain method with the dynamic Smagorinsky
-model on the synthetic free-shear-layer case at 192x128x96 cells -- tanh
-mean profile U(y) = Um + dU/2 tanh(2y/delta_w0), velocity ratio U2/U1 = 0.5,
-convective Mach 0.3, Re 500 on delta_w0 = 4 dy, seeded 2 % dU perturbations
-where |y| < delta_w0, acoustic CFL 0.5, 3-stage TVD RK.  Inputs are resident
-in HBM before the timed region.
+Default workload: the FP32 + learned-closure path (mixed precision: FP32
+stencil arithmetic, FP64 state / pressure recovery / RK / CFL; the per-cell
+BF16-MFMA network for Cs^2, judged against the dynamic model every 50
+steps with a dynamic fallback above 10 % error) on the synthetic free shear
+layer at 384x256x256 cells, Mc 0.4 -- tanh mean profile U(y) = Um + dU/2
+tanh(2y/delta_w0), U2/U1 = 0.5, Re 500 on delta_w0 = 4 dy, seeded 2 % dU
+perturbations where |y| < delta_w0, acoustic CFL 0.5, 3-stage TVD RK --
+timed beside the plain FP64 dynamic-Smagorinsky path on the same grid and
+initial state.  Inputs are resident in HBM before each timed region.
 
 Metric: Mcell-updates/s per RK stage = cells x stages x steps / seconds.
 
-Prints one JSON line (see the keys below).  `roofline` prices the dominant
-kernel's algorithmic bytes per launch against its average launch duration,
-both measured with HIP events on the context's stream over the timed steps.
-`cpu_baseline` times the FP64 CPU oracle on one host core on a bounded
-sample (two steps of the same workload) -- a reported baseline, not the
-target.
+Prints one JSON line.  `value` is the headline path's throughput;
+`fp64_dynamic` the comparison leg and `speedup_vs_fp64_dynamic` their
+ratio.  `roofline` prices the dominant kernel's algorithmic bytes per
+launch against its average launch duration, measured with HIP events on the
+context's stream over the timed steps; `mlp_roofline` the learned-closure
+kernel's MFMA FLOPs against the BF16 dense peak.  `cpu_baseline` times the
+FP64 CPU oracle (the reference algorithm, one host core) on a bounded
+sample -- a reported baseline, not the target.
 
-    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--mesh 192x128x96]
-                    [--sgs dynamic|constant] [--precision fp64|mixed]
-                    [--path fused|unfused] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--mesh 384x256x256]
+                    [--sgs learned|dynamic|constant] [--precision mixed|fp64]
+                    [--mach 0.4] [--path fused|unfused] [--no-compare] [--no-cpu]
 """
 import argparse
 import json
@@ -33,16 +38,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import freeshearlayer_amd as fsl  # noqa: E402
+from freeshearlayer_amd import shear_case  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MODEL = os.path.join(ROOT, "freeshearlayer_amd", "data", "sgs_mlp.bin")
+METRIC = "Mcell-updates/s per RK stage on 1 MI355X, FP32+learned SGS vs FP64 dyn-Smag"
+UNIT = "Mcell-updates/s per RK stage"
 
 
 def algorithmic_bytes(kernel, L, H, D, sgs):
-    """Compulsory HBM bytes of one launch of `kernel` (FP64 state):
-    each cell-sized array streamed once per access pattern it needs."""
+    """Compulsory HBM bytes of one launch of `kernel` (FP64 state): each
+    cell-sized array streamed once per access pattern it needs."""
     N = L * H * D
     G = (L + 2) * (H + 2) * (D + 2)
     nx, ny, nz = (L + 1) * H * D, L * (H + 1) * D, L * H * (D + 1)
+    mu = 8 if sgs else 0
     per = {
         # read U (5 fields), write 30 face states
         "recon": 5 * 8 * N + 30 * 8 * (nx + ny + nz) / 3,
@@ -52,42 +63,83 @@ def algorithmic_bytes(kernel, L, H, D, sgs):
         "flux_z": 10 * 8 * nz + 5 * 8 * G + (8 * G if sgs else 0) + 5 * 8 * nz,
         # read 15 flux arrays, U and Up, write one state
         "rk_update": 5 * 8 * (nx + ny + nz) + 3 * 5 * 8 * N,
-        # fused stage, averaged over the three stages: read the stage state
-        # (40 B/cell), the step state in stages 2-3 (40 B/cell), mu_SGS (8) and
-        # write the new state (40): (88 + 128 + 128) / 3 dynamic, (80+120+120)/3 constant
-        "fused_stage": N * ((40 + 40 + (8 if sgs else 0)) + 2 * (40 + 40 + 40 + (8 if sgs else 0))) / 3.0,
+        # fused stage (FP64 or mixed: both stream FP64 state), averaged over
+        # the three stages: read the stage state (40 B/cell), the step state
+        # in stages 2-3 (40), mu_SGS (8), write the new state (40)
+        "fused_stage": N * ((40 + 40 + mu) + 2 * (40 + 40 + 40 + mu)) / 3.0,
         "filter_x": 2 * 8 * G, "filter_y": 2 * 8 * G, "filter_z": 2 * 8 * G,
     }
     return per.get(kernel)
 
 
-def make_context(L, H, D, sgs, precision, path, cfl):
-    case = fsl.case_reference(L, H, D)
-    shear = fsl.shear_defaults()
-    U, u_in, mu_L = fsl.ic_shear_layer(case, shear)
-    case.mu_L = mu_L
-    case.Pr_L = 0.72
-    case.nStages = 3
-    case.BL_HIG = min(5, H - H // 2)
-    cfg = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC if sgs == "dynamic" else fsl.FSL_SGS_CONSTANT,
-                     precision=fsl.FSL_PREC_FP64 if precision == "fp64" else fsl.FSL_PREC_MIXED,
-                     cfl_mode=fsl.FSL_CFL_ACOUSTIC, cfl=cfl, fused=1 if path == "fused" else 0,
-                     rho_in=shear.rho0, p_in=shear.p0, p_out=shear.p0)
-    ctx = fsl.Context(case, cfg)
-    ctx.set_inflow(u_in)
-    ctx.upload_state(U)
-    ctx.set_lcg(0.317391239817, 0)
-    return ctx, case, U, u_in
+# MFMA work of the learned closure per cell: 64-cell tiles through
+# 16x64 (K padded 8 -> 16), 64x64 and 64x16 (w3 in column 0) products
+MLP_FLOP_PER_CELL = 2 * (16 * 64 + 64 * 64 + 64 * 16)
 
 
-def cpu_baseline(L, H, D, sgs, U, u_in, case, steps=2):
-    """FP64 CPU oracle (oracle/liboracle.so, scalar C, one core) on two steps
-    of the same workload."""
+def workload_name(precision, sgs, path, L, H, D, mach):
+    return "%s %s-SGS %s path, %dx%dx%d cells, Mc %.1f, 3-stage TVD RK" % (
+        "FP32+FP64" if precision == "mixed" else "FP64", sgs, path, L, H, D, mach)
+
+
+def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=True):
+    ctx, case, U0, u_in, _ = shear_case.make(L, H, D, sgs=sgs, precision=precision, fused=(path == "fused"),
+                                             cfl=cfl, mach=mach)
+    if sgs == "learned":
+        p, n = fsl.mlp_file_read(MODEL)
+        ctx.mlp_set_params(p, n)
+        ctx.learned_configure(50, 0.10)
+    for _ in range(warmup):
+        ctx.step()
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(profile)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(steps)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    ctx.profile(False)
+    prof = ctx.profile_read()
+    sums = ctx.conservation_sums()
+    status = ctx.learned_status() if sgs == "learned" else None
+    out = {"ctx": ctx, "case": case, "U0": U0, "u_in": u_in, "t": t, "prof": prof, "sums": sums,
+           "status": status, "value": L * H * D * 3 * steps / t / 1e6}
+    return out
+
+
+def roofline_of(prof, L, H, D, sgs, workload):
+    if not prof:
+        return None, {}
+    kernels = {k: {"ms_total": round(v[0], 3), "launches": int(v[1]), "avg_us": round(v[0] / v[1] * 1e3, 2)}
+               for k, v in sorted(prof.items())}
+    name, (ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
+    b = algorithmic_bytes(name, L, H, D, sgs != "constant")
+    if b is None:
+        return None, kernels
+    avg_s = ms / cnt / 1e3
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("%s@%s" % (name, workload))
+    ach = b / avg_s / 1e9
+    return ({"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
+             "alg_bytes_per_launch": int(b)}, kernels)
+
+
+def cpu_baseline(steps=2, mesh=(192, 128, 96)):
+    """The FP64 dynamic-SGS CPU oracle (oracle/liboracle.so: the reference's
+    serial algorithm in scalar C, one core) on a bounded sample: `steps`
+    steps of the same shear layer on a smaller mesh (the per-cell cost does
+    not depend on the mesh size)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
-    orc = Oracle(L, H, D, dynamic=int(sgs == "dynamic"))
-    orc.set("mu_L", case.mu_L)
-    orc.set("BL_HIG", case.BL_HIG)
+    L, H, D = mesh
+    c, U, u_in, _ = shear_case.initial(L, H, D, mach=0.4)
+    orc = Oracle(L, H, D, dynamic=1)
+    orc.set("mu_L", c.mu_L)
+    orc.set("BL_HIG", c.BL_HIG)
     orc.u_in()[:] = u_in
     orc.derive()
     orc.set_state(U)
@@ -97,23 +149,24 @@ def cpu_baseline(L, H, D, sgs, U, u_in, case, steps=2):
         orc.step()
     t = time.perf_counter() - t0
     orc.close()
-    return {"value": L * H * D * 3 * steps / t / 1e6, "unit": "Mcell-updates/s per RK stage",
-            "cores": 1, "kind": "port",
-            "sample": "%d steps of the FP64 %s-SGS CPU oracle at %dx%dx%d (%.1f s)"
-                      % (steps, sgs, L, H, D, t),
+    return {"value": round(L * H * D * 3 * steps / t / 1e6, 4), "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": "%d steps of the FP64 dynamic-SGS CPU oracle on the same shear layer at %dx%dx%d "
+                      "(%.1f s)" % (steps, L, H, D, t),
             "cpu": platform.processor() or platform.machine()}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mesh", default="192x128x96")
-    ap.add_argument("--sgs", default="dynamic", choices=["dynamic", "constant"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--mesh", default="384x256x256")
+    ap.add_argument("--sgs", default="learned", choices=["learned", "dynamic", "constant"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--path", default="fused", choices=["fused", "unfused"])
+    ap.add_argument("--mach", type=float, default=0.4)
     ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 dynamic comparison leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
@@ -123,8 +176,7 @@ def main(argv=None):
         ctx = fsl.Context(fsl.case_reference(8, 10, 8), fsl.config(sgs_model=fsl.FSL_SGS_CONSTANT))
         ctx.profile(True)
         ctx.copy_kernel(a.copy_calibration, 5)
-        prof = ctx.profile_read()
-        ms, cnt = prof["copy"]
+        ms, cnt = ctx.profile_read()["copy"]
         n = a.copy_calibration
         print(json.dumps({"copy_doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n,
                           "avg_us": ms / cnt * 1e3, "GBps": 16 * n / (ms / cnt / 1e3) / 1e9}))
@@ -134,74 +186,59 @@ def main(argv=None):
         print("bench.py: this build runs on one GPU; the domain is not sharded across GPUs "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         return 2
-    L, H, D = (int(v) for v in a.mesh.lower().split("x"))
-    ctx, case, U0, u_in = make_context(L, H, D, a.sgs, a.precision, a.path, a.cfl)
-    for _ in range(a.warmup):
-        ctx.step()
-    ctx.synchronize()
-    ctx.profile_reset()
-    if not a.no_profile:
-        ctx.profile(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.run(a.steps)
-    ctx.synchronize()
-    t = time.perf_counter() - t0
-    ctx.profile(False)
-    prof = ctx.profile_read()
-    sums = ctx.conservation_sums()
-    if not np.all(np.isfinite(sums)):
+    L, H, D = shear_case.parse_mesh(a.mesh)
+    wl = workload_name(a.precision, a.sgs, a.path, L, H, D, a.mach)
+    print("[bench] %s: %d warmup + %d timed steps" % (wl, a.warmup, a.steps), file=sys.stderr, flush=True)
+    leg = time_leg(L, H, D, a.sgs, a.precision, a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile)
+    ctx = leg["ctx"]
+    if not np.all(np.isfinite(leg["sums"])):
         print("bench.py: non-finite state after the timed steps", file=sys.stderr)
         return 1
-    N = L * H * D
-    value = N * 3 * a.steps / t / 1e6
-    roofline = None
-    if prof:
-        dom = max(prof.items(), key=lambda kv: kv[1][0])
-        name, (ms, cnt) = dom
-        avg_s = ms / cnt / 1e3
-        b = algorithmic_bytes(name, L, H, D, a.sgs == "dynamic")
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-        wl = "%s %s-SGS %s path, %dx%dx%d cells, 3-stage TVD RK" % (a.precision.upper(), a.sgs, a.path,
-                                                                     L, H, D)
-        if os.path.exists(tpath):
-            traffic = json.load(open(tpath)).get("%s@%s" % (name, wl))
-        if b is not None:
-            ach = b / avg_s / 1e9
-            roofline = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
-                        "alg_bytes_per_launch": int(b)}
-        kernels = {k: {"ms_total": round(v[0], 3), "launches": int(v[1]),
-                       "avg_us": round(v[0] / v[1] * 1e3, 2)} for k, v in sorted(prof.items())}
-    else:
-        kernels = {}
+    roofline, kernels = roofline_of(leg["prof"], L, H, D, a.sgs, wl)
+    mlp_roofline = None
+    if "mlp_mu" in leg["prof"]:
+        ms, cnt = leg["prof"]["mlp_mu"]
+        tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
+        mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
+                        "flop_per_cell": MLP_FLOP_PER_CELL, "avg_launch_us": round(ms / cnt * 1e3, 2)}
+    final_dt = ctx.dt
+    ctx.close()
+    compare = None
+    if not a.no_compare and not (a.sgs == "dynamic" and a.precision == "fp64"):
+        print("[bench] comparison leg: FP64 dynamic-SGS, same grid", file=sys.stderr, flush=True)
+        ref = time_leg(L, H, D, "dynamic", "fp64", a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile)
+        rl, _ = roofline_of(ref["prof"], L, H, D, "dynamic",
+                            workload_name("fp64", "dynamic", a.path, L, H, D, a.mach))
+        compare = {"value": round(ref["value"], 2), "ms_per_step": round(ref["t"] / a.steps * 1e3, 4),
+                   "workload": workload_name("fp64", "dynamic", a.path, L, H, D, a.mach), "roofline": rl}
+        ref["ctx"].close()
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(L, H, D, a.sgs, U0, u_in, case)
+        print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
     line = {
-        "metric": "Mcell-updates/s per RK stage on 1 MI355X, FP32+learned SGS vs FP64 dyn-Smag",
-        "value": round(value, 2),
-        "unit": "Mcell-updates/s per RK stage",
+        "metric": METRIC,
+        "value": round(leg["value"], 2),
+        "unit": UNIT,
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(t / a.steps * 1e3, 4),
+        "ms_per_step": round(leg["t"] / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
-        "data": "synthetic shear layer (tanh profile, Mc 0.3, U2/U1 0.5, Re 500, 2%% seeded "
-                "perturbations, acoustic CFL %.2f)" % a.cfl,
-        "config": {"workload": "%s %s-SGS %s path, %dx%dx%d cells, 3-stage TVD RK"
-                               % (a.precision.upper(), a.sgs, a.path, L, H, D),
-                   "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
-                   "path": a.path},
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64+bf16" if a.sgs == "learned" else "f32+f64",
+        "data": "synthetic shear layer (tanh profile, Mc %.1f, U2/U1 0.5, Re 500, 2%% seeded perturbations, "
+                "acoustic CFL %.2f)" % (a.mach, a.cfl),
+        "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
+                   "path": a.path, "mach": a.mach},
         "roofline": roofline,
+        "mlp_roofline": mlp_roofline,
+        "fp64_dynamic": compare,
+        "speedup_vs_fp64_dynamic": round(leg["value"] / compare["value"], 3) if compare else None,
+        "learned_status": leg["status"],
         "cpu_baseline": cpu,
         "kernels": kernels,
-        "final_dt": ctx.dt,
-        "mass_sum": sums[0],
+        "final_dt": final_dt,
     }
     print(json.dumps(line))
-    ctx.close()
     return 0
 
 

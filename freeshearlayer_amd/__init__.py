@@ -60,7 +60,8 @@ class FslShearStats(C.Structure):
 class FslLearnedStatus(C.Structure):
     _fields_ = [("interval", C.c_int), ("tol", C.c_double), ("in_fallback", C.c_int),
                 ("last_error", C.c_double), ("judged", C.c_longlong), ("fallbacks", C.c_longlong),
-                ("learned_steps", C.c_longlong), ("dynamic_steps", C.c_longlong)]
+                ("learned_steps", C.c_longlong), ("dynamic_steps", C.c_longlong), ("scale", C.c_double),
+                ("adapt", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -173,7 +174,10 @@ def load():
         "fsl_mlp_fit_norm": (i_, [vp, _fp]),
         "fsl_mlp_train": (i_, [vp, i_, ll, C.c_float, C.c_ulonglong, _fp]),
         "fsl_mlp_gradient": (i_, [vp, ll, _fp, _fp]),
+        "fsl_mlp_set_loss_weight": (i_, [vp, C.c_float]),
         "fsl_learned_configure": (i_, [vp, i_, d_]),
+        "fsl_learned_set_scale": (i_, [vp, d_, i_]),
+        "fsl_learned_history": (i_, [vp, i_, _dp, _dp]),
         "fsl_learned_get_status": (i_, [vp, C.POINTER(FslLearnedStatus)]),
         "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
         "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
@@ -478,6 +482,9 @@ class Context:
         self._chk(self.lib.fsl_mlp_train(self.h, epochs, batch, lr, seed, _fptr(loss)), "mlp_train")
         return loss[:epochs]
 
+    def mlp_set_loss_weight(self, power):
+        self._chk(self.lib.fsl_mlp_set_loss_weight(self.h, power), "mlp_set_loss_weight")
+
     def mlp_gradient(self, batch):
         g = np.zeros(MLP_NPARAM, np.float32)
         loss = np.zeros(1, np.float32)
@@ -486,6 +493,16 @@ class Context:
 
     def learned_configure(self, interval=50, tol=0.10):
         self._chk(self.lib.fsl_learned_configure(self.h, interval, tol), "learned_configure")
+
+    def learned_set_scale(self, scale=1.0, adapt=True):
+        self._chk(self.lib.fsl_learned_set_scale(self.h, scale, int(adapt)), "learned_set_scale")
+
+    def learned_history(self):
+        n = self.lib.fsl_learned_history(self.h, 0, None, None)
+        e = np.zeros(max(n, 1))
+        sc = np.zeros(max(n, 1))
+        self.lib.fsl_learned_history(self.h, n, _ptr(e), _ptr(sc))
+        return e[:n], sc[:n]
 
     def learned_status(self):
         s = FslLearnedStatus()

@@ -28,6 +28,7 @@ struct LearnedState {
     float* adam_v = nullptr;
     float* loss = nullptr;
     int adam_t = 0;
+    float wpow = 0.f, winv = 1.f;   // loss weight (feature 6)^wpow, normalised to mean 1
     int ready = 0;             // parameters set
     float* X = nullptr;        // sample store: features (8 per sample), targets
     float* t = nullptr;
@@ -42,7 +43,10 @@ struct LearnedState {
     int interval = 50;         // judge every interval steps (0: never)
     double tol = 0.10;         // relative dissipation error that triggers the fallback
     int in_fallback = 0;
+    int adapt = 1;             // judge rescales the network's Cd to the dynamic model's dissipation
+    float scale = 1.f;         // current output scale of the network's Cd
     double last_error = -1.;
+    std::vector<float> hist_err, hist_scale;   // per judge: error, scale in use
     long long judged = 0, fallbacks = 0, learned_steps = 0, dynamic_steps = 0;
 };
 

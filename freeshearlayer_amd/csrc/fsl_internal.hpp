@@ -84,13 +84,14 @@ void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* 
                         const float* norm, float* y);
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                    double dy, const void* pack, const float* master, const float* norm, double* mu,
-                   float* cd_out);
+                   float* cd_out, float scale);
 void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                          double dy, const float* cd, long long stride, long long offset, float* X, float* t,
                          long long nout);
 void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
                            long long step, unsigned long long pa, unsigned long long pc, const void* pack,
-                           const float* master, const float* norm, float* grad, float* loss);
+                           const float* master, const float* norm, float* grad, float* loss, float wpow,
+                           float winv);
 void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t);
 void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D);
 int dissipation_nblocks();

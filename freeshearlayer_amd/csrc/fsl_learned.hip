@@ -121,7 +121,7 @@ int learned_mu(fsl_ctx* c, double* mu)
     }
     prof_mark(c, K_MLP_MU, 0);
     launch_mlp_mu(c->stream, cst5(c->U, c->g.G), c->g, c->ph, c->dU, L.dw, c->cs.deltaY, L.pack, L.master,
-                  L.norm, mu, nullptr);
+                  L.norm, mu, nullptr, L.scale);
     prof_mark(c, K_MLP_MU, 1);
     return check_launch(c, "learned mu_SGS");
 }
@@ -215,6 +215,12 @@ int learned_sgs_step(fsl_ctx* c)
         if ((rc = judge_error(c, c->mu, L.mu_alt, &err, &dd, &dl))) return rc;
         L.judged++;
         L.last_error = err;
+        L.hist_err.push_back((float)err);
+        L.hist_scale.push_back(L.scale);
+        // global level of the network's coefficient from the dynamic model
+        // (the network supplies the spatial distribution): used from the next step
+        const float next = (L.adapt && dl > 0 && dd > 0) ? std::min(10.f, std::max(0.1f, (float)(L.scale * dd / dl)))
+                                                         : L.scale;
         if (err > L.tol) {
             L.fallbacks++;
             L.in_fallback = 1;
@@ -224,6 +230,7 @@ int learned_sgs_step(fsl_ctx* c)
             L.in_fallback = 0;
             L.learned_steps++;
         }
+        L.scale = next;
     } else if (L.in_fallback) {
         if ((rc = dynamic_mu(c, nullptr))) return rc;
         L.dynamic_steps++;
@@ -256,6 +263,7 @@ int fsl_mlp_set_params(fsl_ctx* c, const float* params, const float* norm)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     L.ready = 1;
     L.adam_t = 0;
+    L.scale = 1.f;   // the judge recalibrates the new network from scratch
     if (L.grad) {   // a new parameter set restarts the optimizer
         const size_t nb = mlp_nparam() * sizeof(float);
         HIPCHK(c, hipMemsetAsync(L.adam_m, 0, nb, c->stream));
@@ -455,6 +463,32 @@ int fsl_mlp_load_samples(fsl_ctx* c, const float* X, const float* t, long long n
     return FSL_OK;
 }
 
+// loss weight (|S| Delta / dU)^power of each sample, normalised to mean 1 over the store
+int fsl_mlp_set_loss_weight(fsl_ctx* c, float power)
+{
+    LearnedState& L = c->lrn;
+    if (!(power >= 0.f) || power > 8.f) return set_err(c, FSL_ERR_ARG, "weight power %g", (double)power);
+    if (power == 0.f) {
+        L.wpow = 0.f;
+        L.winv = 1.f;
+        return FSL_OK;
+    }
+    if (L.n < 1) return set_err(c, FSL_ERR_ARG, "need samples");
+    const long long chunk = 1 << 20;
+    std::vector<float> buf((size_t)chunk * 8);
+    double sw = 0;
+    for (long long f = 0; f < L.n; f += chunk) {
+        const long long m = std::min(chunk, L.n - f);
+        int rc = fsl_mlp_samples(c, f, m, buf.data(), nullptr);
+        if (rc) return rc;
+        for (long long r = 0; r < m; r++) sw += pow(std::max((double)buf[r * 8 + 6], 0.), (double)power);
+    }
+    if (!(sw > 0)) return set_err(c, FSL_ERR_ARG, "all weights vanish");
+    L.wpow = power;
+    L.winv = (float)(L.n / sw);
+    return FSL_OK;
+}
+
 // feature means and reciprocal standard deviations of the stored samples
 int fsl_mlp_fit_norm(fsl_ctx* c, float* norm)
 {
@@ -502,7 +536,7 @@ int fsl_mlp_train(fsl_ctx* c, int epochs, long long batch, float lr, unsigned lo
         prof_mark(c, K_MLP_TRAIN, 0);
         for (long long s = 0; s < steps; s++) {
             launch_mlp_train_step(c->stream, L.X, L.t, L.n, batch, s, a, b, L.pack, L.master, L.norm, L.grad,
-                                  L.loss);
+                                  L.loss, L.wpow, L.winv);
             launch_mlp_adam(c->stream, L.master, L.grad, L.adam_m, L.adam_v, lr, ++L.adam_t);
             launch_mlp_pack(c->stream, L.master, L.pack);
         }
@@ -529,7 +563,8 @@ int fsl_mlp_gradient(fsl_ctx* c, long long batch, float* grad, float* loss)
     HIPCHK(c, hipMemsetAsync(L.grad, 0, mlp_nparam() * sizeof(float), c->stream));
     HIPCHK(c, hipMemsetAsync(L.loss, 0, sizeof(float), c->stream));
     // samples 0 .. batch-1 in order (permutation a = 1, b = 0)
-    launch_mlp_train_step(c->stream, L.X, L.t, L.n, batch, 0, 1, 0, L.pack, L.master, L.norm, L.grad, L.loss);
+    launch_mlp_train_step(c->stream, L.X, L.t, L.n, batch, 0, 1, 0, L.pack, L.master, L.norm, L.grad, L.loss,
+                          L.wpow, L.winv);
     if ((rc = check_launch(c, "gradient"))) return rc;
     HIPCHK(c, hipMemcpyAsync(grad, L.grad, mlp_nparam() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (loss) HIPCHK(c, hipMemcpyAsync(loss, L.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -548,9 +583,30 @@ int fsl_learned_configure(fsl_ctx* c, int interval, double tol)
     return FSL_OK;
 }
 
+int fsl_learned_set_scale(fsl_ctx* c, double scale, int adapt)
+{
+    if (!(scale >= 0.1 && scale <= 10.)) return set_err(c, FSL_ERR_ARG, "scale %g outside [0.1, 10]", scale);
+    c->lrn.scale = (float)scale;
+    c->lrn.adapt = adapt ? 1 : 0;
+    return FSL_OK;
+}
+
+int fsl_learned_history(const fsl_ctx* c, int nmax, double* err, double* scale)
+{
+    const LearnedState& L = c->lrn;
+    const int n = (int)L.hist_err.size();
+    for (int q = 0; q < n && q < nmax; q++) {
+        if (err) err[q] = L.hist_err[q];
+        if (scale) scale[q] = L.hist_scale[q];
+    }
+    return n;
+}
+
 int fsl_learned_get_status(const fsl_ctx* c, fsl_learned_status* s)
 {
     const LearnedState& L = c->lrn;
+    s->scale = L.scale;
+    s->adapt = L.adapt;
     s->interval = L.interval;
     s->tol = L.tol;
     s->in_fallback = L.in_fallback;

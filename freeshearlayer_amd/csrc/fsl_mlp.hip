@@ -1,7 +1,8 @@
 // fsl_mlp.hip -- learned SGS closure: a per-cell MLP (8 -> 64 -> 64 -> 1,
 // tanh) that predicts the dynamic-Smagorinsky coefficient Cd = Cs^2 from
-// normalised invariants of the resolved strain and rotation, |S| Delta and
-// the distance from the layer centre, replacing the test filter and the
+// normalised invariants of the resolved strain and rotation, a grid-scale
+// roughness ratio, |S| Delta and the distance from the layer centre,
+// replacing the test filter and the
 // Germano-Lilly least squares of src/turbulence.c:103-507.  mu_SGS is then
 // rho Cd Delta^2 |S| as in src/turbulence.c:442.
 //
@@ -14,6 +15,9 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
+
+#include <algorithm>
 
 #include "fsl_internal.hpp"
 
@@ -40,24 +44,30 @@ constexpr int PB2T = PB2 + 16 * 64;     // W2 transposed (backward): 4 k-steps x
 constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
 constexpr int NPACK = PB3 + 4 * 64;
 
-__device__ __forceinline__ unsigned short f2bf(float f)
+typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+
+__device__ __forceinline__ bf4 cvt4(float a, float b, float c, float d)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
-    return (unsigned short)(u >> 16);
+    const f4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf4, __builtin_convertvector(v, bfv4));
 }
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ float tanh_fast(float x)
+// tanh(z + b) = 1 - 2 / (2^(K (z + b)) + 1), K = 2 log2(e), with kb = K b:
+// one FMA for the scaled argument, v_exp_f32, v_rcp_f32 and one FMA.  No
+// clamp: 2^u saturates to +inf or 0 and the reciprocal maps those to +-1.
+// BF16 activations keep 8 bits, far coarser than this evaluation's error.
+constexpr float kTanhK = 2.8853900817779268f;
+__device__ __forceinline__ float act(float z, float kb)
 {
-    // tanh x = 1 - 2 / (exp(2x) + 1), exp through v_exp_f32 (2^x); BF16
-    // activations keep 8 bits, far coarser than this evaluation's error
-    const float xc = fminf(fmaxf(x, -15.f), 15.f);
-    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * xc);   // 2 log2(e) x
-    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+    const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(z, kTanhK, kb));
+    return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(e + 1.f), 1.f);
 }
 
 struct MlpW {
@@ -93,9 +103,9 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         for (int tn = 0; tn < 4; tn++) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = mfma(a, w.pack[(PB1 >> 6) * 64 + tn * 64 + lane], acc);
-            const float b = w.master[OB1 + tn * 16 + r16];
+            const float kb = kTanhK * w.master[OB1 + tn * 16 + r16];
 #pragma unroll
-            for (int e = 0; e < 4; e++) ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(tanh_fast(acc[e] + b));
+            for (int e = 0; e < 4; e++) ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(act(acc[e], kb));
         }
     }
     __syncthreads();
@@ -108,10 +118,10 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
             for (int tn = 0; tn < 4; tn++) acc[tn] = mfma(a, w.pack[PB2 / 64 * 64 + (ks * 4 + tn) * 64 + lane], acc[tn]);
         }
         for (int tn = 0; tn < 4; tn++) {
-            const float b = w.master[OB2 + tn * 16 + r16];
+            const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(tanh_fast(acc[tn][e] + b));
+                ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(act(acc[tn][e], kb));
         }
     }
     __syncthreads();
@@ -140,22 +150,33 @@ struct FeatArgs {
     double dy;
 };
 
-// Velocity-gradient features of interior cell (i, j, k); returns |S| =
-// sqrt(2 S:S) (the model's strain magnitude, src/turbulence.c:299-301).
+// Features of interior cell (i, j, k) from the 7-point stencil of u =
+// (rho u) / rho: invariants of the central-difference strain S and rotation
+// W, the grid-scale roughness Delta |lap u| / |S| (second differences), |S|
+// Delta / dU and y / delta_w(x).  Returns |S| = sqrt(2 S:S) (the model's
+// strain magnitude, src/turbulence.c:299-301).
 __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, int k, float f[NIN])
 {
     const Geo& g = a.g;
     const long long c = cidx(g, i, j, k);
     const long long st[3] = {g.si, g.sj, 1};
-    float gr[3][3];
+    const float rc = __builtin_amdgcn_rcpf((float)a.U.f[0][c]);
+    float uc[3], gr[3][3], lap[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 3; r++) uc[r] = (float)a.U.f[1 + r][c] * rc;
 #pragma unroll
     for (int e = 0; e < 3; e++) {
         const long long p = c + st[e], m = c - st[e];
         const float rp = __builtin_amdgcn_rcpf((float)a.U.f[0][p]);
         const float rm = __builtin_amdgcn_rcpf((float)a.U.f[0][m]);
         const float h = (float)a.ph.r2del[e];
+        const float h2 = (float)(a.ph.rdel[e] * a.ph.rdel[e]);
 #pragma unroll
-        for (int r = 0; r < 3; r++) gr[r][e] = ((float)a.U.f[1 + r][p] * rp - (float)a.U.f[1 + r][m] * rm) * h;
+        for (int r = 0; r < 3; r++) {
+            const float up = (float)a.U.f[1 + r][p] * rp, um = (float)a.U.f[1 + r][m] * rm;
+            gr[r][e] = (up - um) * h;
+            lap[r] += (up - 2.f * uc[r] + um) * h2;
+        }
     }
     float S[3][3], W[3][3];
 #pragma unroll
@@ -202,13 +223,15 @@ __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, 
         for (int e = 0; e < 3; e++) ws += om[r] * S[r][e] * om[e];
     }
     const float yv = ((float)j - (float)(g.H / 2) - 0.5f) * (float)a.dy;
-    f[0] = (w2 - s2) / (w2 + s2 + tiny);
-    f[1] = s3 / (s2 * sn + tiny);
-    f[2] = sww / (sn * w2 + tiny);
-    f[3] = ssww / (s2 * w2 + tiny);
-    f[4] = (gr[0][0] + gr[1][1] + gr[2][2]) / (smag + tiny);
-    f[5] = ws / (oo * smag + tiny);
-    f[6] = smag * sqrtf((float)a.ph.DD) / (float)a.dU;
+    // ratios through v_rcp_f32 (1 ulp): the network sees them rounded to BF16
+    f[0] = (w2 - s2) * __builtin_amdgcn_rcpf(w2 + s2 + tiny);
+    f[1] = s3 * __builtin_amdgcn_rcpf(s2 * sn + tiny);
+    f[2] = sww * __builtin_amdgcn_rcpf(sn * w2 + tiny);
+    f[3] = ssww * __builtin_amdgcn_rcpf(s2 * w2 + tiny);
+    const float delta = sqrtf((float)a.ph.DD);   // (dx dy dz)^(1/3)
+    f[4] = sqrtf(lap[0] * lap[0] + lap[1] * lap[1] + lap[2] * lap[2]) * delta * __builtin_amdgcn_rcpf(smag + tiny);
+    f[5] = ws * __builtin_amdgcn_rcpf(oo * smag + tiny);
+    f[6] = smag * delta * (float)(1. / a.dU);
     f[7] = yv * __builtin_amdgcn_rcpf(a.dw[i]);
     return smag;
 }
@@ -236,39 +259,126 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
     if (t) t[s] = cd[cidx(a.g, i, j, k)] / 0.15f;
 }
 
-// learned mu_SGS for all interior cells (one wavefront = 64 cells)
-__global__ void __launch_bounds__(256) k_mlp_mu(FeatArgs a, MlpW w, double* mu, float* cd_out)
+// Learned mu_SGS, register-resident form.  The network runs transposed:
+// Z^T = W^T X^T with units as MFMA rows (M) and cells as columns (N).  The
+// output tile D[unit][cell] then sits in the lanes exactly as the next
+// layer's B operand B[unit][cell] (lane = cell + 16 (unit / 4), element =
+// unit % 4), so activations go from the accumulators through bias, tanh and
+// BF16 rounding straight into the next product.  The weight tiles are the
+// A operands (same lane layout as the packed B tiles) and stay in registers
+// for the block's whole grid-stride loop; only the 8 features of each cell
+// pass through LDS (one 8-byte read per lane and 16-cell tile).
+constexpr int MU_CELLS = 256;   // cells per block iteration (4 waves x 64)
+
+template <int TP>   // 16-cell tiles in flight per layer
+__global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double* mu, float* cd_out, float scale)
 {
-    __shared__ WaveLds lds[4];
+    __shared__ unsigned short sX[4][64 * NIN];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveLds& ws = lds[wv];
-    const long long N = (long long)a.g.L * a.g.H * a.g.D;
-    const long long cell = (long long)blockIdx.x * 256 + threadIdx.x;
-    float f[NIN], smag = 0.f;
-    int i = 1, j = 1, k = 1;
-    const bool valid = cell < N;
-    if (valid) {
-        decode_interior(a.g, cell, i, j, k);
-        smag = cell_features(a, i, j, k, f);
-    } else {
+    const int r16 = lane & 15, q4 = lane >> 4;
+    bf4 A1[4], A2[16], A3[4];
+    float kb1[4][4], kb2[4][4];
 #pragma unroll
-        for (int q = 0; q < NIN; q++) f[q] = 0.f;
+    for (int t = 0; t < 4; t++) {
+        A1[t] = w.pack[PB1 + t * 64 + lane];
+        A3[t] = w.pack[PB3 + t * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            kb1[t][e] = kTanhK * w.master[OB1 + t * 16 + 4 * q4 + e];
+            kb2[t][e] = kTanhK * w.master[OB2 + t * 16 + 4 * q4 + e];
+        }
     }
 #pragma unroll
-    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = f2bf((f[q] - w.norm[q]) * w.norm[NIN + q]);
+    for (int t = 0; t < 16; t++) A2[t] = w.pack[PB2 + t * 64 + lane];
+    const float b3 = w.master[OB3];
+    float nm[NIN], ns[NIN];
 #pragma unroll
-    for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
-    __syncthreads();
-    forward64(ws, w, lane);
-    if (valid) {
-        const float Cd = fminf(fmaxf(0.15f * ws.y[lane], 0.f), 0.15f);
-        const long long c = cidx(a.g, i, j, k);
-        mu[c] = a.U.f[0][c] * (double)Cd * a.ph.DD * (double)smag;
-        if (cd_out) cd_out[c] = Cd;
+    for (int q = 0; q < NIN; q++) {
+        nm[q] = w.norm[q];
+        ns[q] = w.norm[NIN + q];
+    }
+    const long long N = (long long)a.g.L * a.g.H * a.g.D;
+    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
+    unsigned short* X = sX[wv];
+    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const long long cell = blk * MU_CELLS + threadIdx.x;
+        const bool valid = cell < N;
+        float f[NIN], smag = 0.f;
+        int i = 1, j = 1, k = 1;
+        if (valid) {
+            decode_interior(a.g, cell, i, j, k);
+            smag = cell_features(a, i, j, k, f);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NIN; q++) f[q] = nm[q];
+        }
+        float fn[NIN];
+#pragma unroll
+        for (int q = 0; q < NIN; q++) fn[q] = (f[q] - nm[q]) * ns[q];
+        *(bf4*)&X[lane * NIN] = cvt4(fn[0], fn[1], fn[2], fn[3]);
+        *(bf4*)&X[lane * NIN + 4] = cvt4(fn[4], fn[5], fn[6], fn[7]);
+        __syncthreads();
+        float ymine = 0.f;
+#pragma unroll
+        for (int s0 = 0; s0 < 4; s0 += TP) {
+            // B operand of layer 1: features 4 q4 .. 4 q4 + 3 of cell st * 16 + r16 (K padded to 16)
+            bf4 xb[TP];
+#pragma unroll
+            for (int u = 0; u < TP; u++) {
+                xb[u] = bf4{0, 0, 0, 0};
+                if (q4 < 2) xb[u] = *(const bf4*)&X[((s0 + u) * 16 + r16) * NIN + 4 * q4];
+            }
+            bf4 h1[TP][4], h2[TP][4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int u = 0; u < TP; u++) {
+                    const f4 z = mfma(A1[t], xb[u], f4{0.f, 0.f, 0.f, 0.f});
+                    h1[u][t] = cvt4(act(z[0], kb1[t][0]), act(z[1], kb1[t][1]), act(z[2], kb1[t][2]),
+                                    act(z[3], kb1[t][3]));
+                }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int u = 0; u < TP; u++) {
+                    f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++) z = mfma(A2[ks * 4 + t], h1[u][ks], z);
+                    h2[u][t] = cvt4(act(z[0], kb2[t][0]), act(z[1], kb2[t][1]), act(z[2], kb2[t][2]),
+                                    act(z[3], kb2[t][3]));
+                }
+#pragma unroll
+            for (int u = 0; u < TP; u++) {
+                f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) z = mfma(A3[ks], h2[u][ks], z);
+                // row 0 of the output tile: y of cell st * 16 + r16 sits in lane r16, element 0
+                const float yt = __shfl(z[0], r16, 64);
+                if (q4 == s0 + u) ymine = yt + b3;
+            }
+        }
+        if (valid) {
+            const float Cd = fminf(fmaxf(0.15f * scale * ymine, 0.f), 0.15f);
+            const long long c = cidx(a.g, i, j, k);
+            mu[c] = a.U.f[0][c] * (double)Cd * a.ph.DD * (double)smag;
+            if (cd_out) cd_out[c] = Cd;
+        }
+        __syncthreads();   // X is rewritten by the next iteration
     }
 }
 
-// forward pass on caller-given raw features (normalised here as in k_mlp_mu)
+int mu_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("FSL_MLP_TILES");
+        v = e ? atoi(e) : 2;
+        if (v != 1 && v != 2 && v != 4) v = 2;
+    }
+    return v;
+}
+
+// forward pass on caller-given raw features (normalised as in k_mlp_mu_tr)
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, MlpW w, float* y)
 {
     __shared__ WaveLds lds[4];
@@ -293,7 +403,8 @@ struct TrainArgs {
     long long step;      // step index inside the epoch
     unsigned long long perm_a, perm_c;   // sample permutation: (a * idx + c) mod n
     float* grad;         // NPARAM accumulators
-    float* loss;         // sum of squared errors
+    float* loss;         // sum of weighted squared errors
+    float wpow, winv;    // sample weight (|S| Delta / dU)^wpow * winv (feature 6); wpow 0: 1
 };
 
 __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
@@ -313,10 +424,11 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
     forward64(ws, w, lane);
+    const float wgt = T.wpow != 0.f ? powf(fmaxf(T.X[s * NIN + 6], 0.f), T.wpow) * T.winv : 1.f;
     const float err = ws.y[lane] - T.t[s];
-    const float dy = err * (2.f / (float)T.batch);
+    const float dy = wgt * err * (2.f / (float)T.batch);
     sdy[wv][lane] = dy;
-    atomicAdd(T.loss, err * err);
+    atomicAdd(T.loss, wgt * err * err);
     __syncthreads();
     // layer 3: grad w3[k] = sum_s H2[s][k] dy[s], grad b3; dZ2 = dy w3 (1 - H2^2)
     {
@@ -612,7 +724,8 @@ void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* 
 }
 
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
-                   double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out)
+                   double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out,
+                   float scale)
 {
     FeatArgs a;
     a.U = U;
@@ -622,8 +735,21 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     a.dw = dw;
     a.dy = dy;
     const long long N = (long long)g.L * g.H * g.D;
-    hipLaunchKernelGGL(k_mlp_mu, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a,
-                       mlpw(pack, master, norm), mu, cd_out);
+    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
+    // one resident wave of blocks (3 per CU at this kernel's occupancy) loops over the cells
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const int tp = mu_variant();
+    const dim3 grid((unsigned)std::min<long long>(nblk, 3LL * ncu));
+    const MlpW w = mlpw(pack, master, norm);
+    if (tp == 1) hipLaunchKernelGGL(k_mlp_mu_tr<1>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+    else if (tp == 4) hipLaunchKernelGGL(k_mlp_mu_tr<4>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+    else hipLaunchKernelGGL(k_mlp_mu_tr<2>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
 }
 
 void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
@@ -643,9 +769,12 @@ void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph,
 
 void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
                            long long step, unsigned long long pa, unsigned long long pc, const void* pack,
-                           const float* master, const float* norm, float* grad, float* loss)
+                           const float* master, const float* norm, float* grad, float* loss, float wpow,
+                           float winv)
 {
     TrainArgs T;
+    T.wpow = wpow;
+    T.winv = winv;
     T.X = X;
     T.t = t;
     T.n = n;

@@ -277,11 +277,12 @@ int fsl_shear_statistics(const fsl_case *c, const double *prof, double U1, doubl
    0.15) and mu_SGS = rho Cd Delta^2 |S| (src/turbulence.c:436-445 with the
    network's Cd in place of the Germano-Lilly one).  Features per interior
    cell, from central differences of u = (rho u)/rho (FP32):
-     0 (|W|^2 - |S|^2) / (|W|^2 + |S|^2)    4 div u / |S|
+     0 (|W|^2 - |S|^2) / (|W|^2 + |S|^2)    4 Delta |lap u| / |S| (second differences)
      1 tr(S^3) / |S|_F^3                     5 omega.S.omega / (|omega|^2 |S|)
      2 tr(S W^2) / (|S|_F |W|_F^2)          6 |S| Delta / dU
      3 tr(S^2 W^2) / (|S|_F^2 |W|_F^2)      7 y / delta_w(x)
-   with |S| = sqrt(2 S:S), |S|_F^2 = S:S, dU the inflow velocity difference
+   with |S| = sqrt(2 S:S), |S|_F^2 = S:S, Delta = (dx dy dz)^(1/3), dU the
+   inflow velocity difference
    and delta_w(x) the vorticity thickness of the x plane, refreshed at every
    judge.  Parameters: FP32 vector W1[8][64], b1[64], W2[64][64], b2[64],
    w3[64], b3 (FSL_MLP_NPARAM values); norm[16]: feature means, then
@@ -319,29 +320,42 @@ int fsl_mlp_samples(fsl_ctx *ctx, long long first, long long n, float *X, float 
 int fsl_mlp_load_samples(fsl_ctx *ctx, const float *X, const float *t, long long n);
 /* feature means and reciprocal standard deviations of the store -> norm[16] */
 int fsl_mlp_fit_norm(fsl_ctx *ctx, float *norm);
-/* Adam (0.9, 0.999, 1e-8) on the mean squared error of y against the
+/* Sample weights of the loss: (|S| Delta / dU)^power (feature 6) scaled to
+   mean 1 over the store; power 3 weights samples by their share of the SGS
+   dissipation rho Cd Delta^2 |S|^3.  Default power 0 (unweighted). */
+int fsl_mlp_set_loss_weight(fsl_ctx *ctx, float power);
+/* Adam (0.9, 0.999, 1e-8) on the weighted mean squared error of y against the
    targets; epochs over a fresh permutation each, batch a multiple of 256;
    epoch_loss (epochs values, may be NULL) receives each epoch's mean loss. */
 int fsl_mlp_train(fsl_ctx *ctx, int epochs, long long batch, float lr, unsigned long long seed,
                   float *epoch_loss);
-/* gradient of the mean squared error over samples 0 .. batch-1 (batch a
-   multiple of 256) at the current parameters, without an update; loss
-   receives the sum of squared errors */
+/* gradient of the weighted mean squared error over samples 0 .. batch-1
+   (batch a multiple of 256) at the current parameters, without an update;
+   loss receives the sum of weighted squared errors */
 int fsl_mlp_gradient(fsl_ctx *ctx, long long batch, float *grad, float *loss);
 /* Judge: every interval steps (steps 1, 1 + interval, ...) both models run
    and the relative difference of the SGS dissipation sum(mu_SGS |S|^2) is
    the error; above tol the dynamic model is used, and kept every step until
-   a later judge passes.  Defaults: interval 50, tol 0.10; interval 0 never
-   judges. */
+   a later judge passes.  With adapt on, each judge also sets the network's
+   output scale (Cd = clip(0.15 scale y, 0, 0.15)) to scale D_dynamic /
+   D_learned, clamped to [0.1, 10], for the following steps: the network
+   gives the spatial distribution of Cd, the dynamic procedure its global
+   level.  Defaults: interval 50, tol 0.10, scale 1, adapt on; interval 0
+   never judges. */
 int fsl_learned_configure(fsl_ctx *ctx, int interval, double tol);
+int fsl_learned_set_scale(fsl_ctx *ctx, double scale, int adapt);
 typedef struct fsl_learned_status {
     int interval;
     double tol;
     int in_fallback;
     double last_error;       /* -1 before the first judge */
     long long judged, fallbacks, learned_steps, dynamic_steps;
+    double scale;            /* output scale in use */
+    int adapt;
 } fsl_learned_status;
 int fsl_learned_get_status(const fsl_ctx *ctx, fsl_learned_status *s);
+/* judge history: error and the scale in use at each judge; returns the count */
+int fsl_learned_history(const fsl_ctx *ctx, int nmax, double *err, double *scale);
 /* one judge evaluation on U now (no state change beyond mu_SGS = dynamic) */
 int fsl_learned_judge(fsl_ctx *ctx, double *err, double *d_dynamic, double *d_learned);
 /* one v_mfma_f32_16x16x16_bf16 product D = A B of row-major 16x16 FP32

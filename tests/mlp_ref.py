@@ -43,14 +43,15 @@ def forward(X, params, norm, exact=False):
     return y.astype(np.float32), (x, h1, h2)
 
 
-def gradient(X, t, params, norm):
-    """Gradient of mean((y - t)^2) with the device kernel's rounding steps:
+def gradient(X, t, params, norm, weight=None):
+    """Gradient of mean(w (y - t)^2) with the device kernel's rounding steps:
     BF16 activations and back-propagated errors, FP32 master w3."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
     y, (x, h1, h2) = forward(X, params, norm)
     n = len(t)
     err = (y - t).astype(np.float64)
-    dy = err * (2.0 / n)
+    w = np.ones(n) if weight is None else np.asarray(weight, np.float64)
+    dy = w * err * (2.0 / n)
     g = np.zeros(NPARAM)
     g[OW3:OB3] = h2.T.astype(np.float64) @ dy
     g[OB3] = dy.sum()
@@ -61,28 +62,32 @@ def gradient(X, t, params, norm):
                 (1.0 - h1.astype(np.float64) ** 2)).astype(np.float32))
     g[OB1:OW2] = dz1.sum(0)
     g[OW1:OB1] = (x.T.astype(np.float64) @ dz1).reshape(-1)
-    return g, float((err ** 2).sum())
+    return g, float((w * err ** 2).sum())
 
 
 def velocity_gradient(U, case):
-    """du_r/dx_e by central differences over the interior, shape (L, H, D, 3, 3)."""
+    """du_r/dx_e by central differences over the interior, shape (L, H, D, 3, 3),
+    and the second-difference Laplacian of u_r, shape (L, H, D, 3)."""
     d = (case.deltaX, case.deltaY, case.deltaZ)
     r2 = [2.0 * (1.0 / (4.0 * de)) for de in d]
     vel = U[1:4] / U[0]
-    g = np.zeros(vel.shape[1:] + (3, 3))
     sl = (slice(1, -1),) * 3
     g = np.zeros((case.LEN, case.HIG, case.DEP, 3, 3))
+    lap = np.zeros((case.LEN, case.HIG, case.DEP, 3))
     for e in range(3):
         p = list(sl); m = list(sl)
         p[e] = slice(2, None); m[e] = slice(0, -2)
         for r in range(3):
             g[..., r, e] = (vel[r][tuple(p)] - vel[r][tuple(m)]) * r2[e]
-    return g
+            lap[..., r] += (vel[r][tuple(p)] - 2 * vel[r][sl] + vel[r][tuple(m)]) / d[e] ** 2
+    return g, lap
 
 
 def features(U, case, dU, dw):
     """The 8 features of every interior cell (k fastest), float64, and |S|."""
-    gr = velocity_gradient(U, case).reshape(-1, 3, 3)
+    gr, lap = velocity_gradient(U, case)
+    gr = gr.reshape(-1, 3, 3)
+    lap = lap.reshape(-1, 3)
     S = 0.5 * (gr + gr.transpose(0, 2, 1))
     W = 0.5 * (gr - gr.transpose(0, 2, 1))
     s2 = (S * S).sum((1, 2))
@@ -104,7 +109,7 @@ def features(U, case, dU, dw):
     DD = (case.deltaX * case.deltaY * case.deltaZ) ** 0.666666666666667
     tiny = 1e-30
     f = np.stack([(w2 - s2) / (w2 + s2 + tiny), s3 / (s2 * sn + tiny), sww / (sn * w2 + tiny),
-                  ssww / (s2 * w2 + tiny), (gr[:, 0, 0] + gr[:, 1, 1] + gr[:, 2, 2]) / (smag + tiny),
+                  ssww / (s2 * w2 + tiny), np.sqrt((lap ** 2).sum(1)) * np.sqrt(DD) / (smag + tiny),
                   ws / (oo * smag + tiny), smag * np.sqrt(DD) / dU, y / np.asarray(dw, np.float64)[i]], 1)
     return f, smag
 

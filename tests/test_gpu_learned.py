@@ -91,6 +91,28 @@ def test_gradient_matches_numpy():
         assert cos > 0.999, name
 
 
+def test_weighted_gradient_matches_numpy():
+    rng = np.random.default_rng(8)
+    p, norm = fsl.mlp_init_params(6)
+    X = rng.standard_normal((2048, 8)).astype(np.float32)
+    X[:, 6] = rng.uniform(0.0, 0.5, 2048)   # |S| Delta / dU >= 0
+    t = rng.uniform(0, 1, 2048).astype(np.float32)
+    ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
+    with ctx:
+        ctx.mlp_set_params(p, norm)
+        ctx.mlp_load_samples(X, t)
+        ctx.mlp_set_loss_weight(3.0)
+        g, loss = ctx.mlp_gradient(2048)
+    w = X[:, 6].astype(np.float64) ** 3
+    w /= w.mean()
+    gr, lr = mlp_ref.gradient(X, t, p, norm, weight=w)
+    assert abs(loss - lr) <= 1e-3 * lr
+    cos = g.astype(np.float64) @ gr / (np.linalg.norm(g) * np.linalg.norm(gr))
+    err = np.abs(g - gr).max() / np.abs(gr).max()
+    print("weighted gradient: max rel %.2e, cos %.7f" % (err, cos))
+    assert err < 1e-3 and cos > 0.99999
+
+
 def test_training_reduces_loss():
     rng = np.random.default_rng(4)
     X = rng.standard_normal((65536, 8)).astype(np.float32) * 2 + 1
@@ -269,3 +291,29 @@ def test_collect_and_train_on_solver_data():
     print("losses", loss, "judge error %.3f" % err)
     assert np.all(np.isfinite(loss)) and loss[-1] < loss[0]
     assert np.isfinite(err)
+
+
+def test_judge_rescales_network_level():
+    """With adapt on, a passing or failing judge sets the network's output
+    scale to scale * D_dynamic / D_learned (clamped to [0.1, 10])."""
+    ctx, case, U, u_in = shear_ctx(24, 20, 16, sgs=fsl.FSL_SGS_LEARNED, fused=1)
+    p, _ = fsl.mlp_init_params(29)
+    with ctx:
+        ctx.ghost_fill()
+        X = ctx.mlp_features()
+        norm = np.concatenate([X.mean(0), 1 / (X.std(0) + 1e-12)]).astype(np.float32)
+        ctx.mlp_set_params(p, norm)
+        ctx.learned_configure(interval=1, tol=1e30)
+        err, dd, dl = ctx.learned_judge()          # step-start state, scale 1
+        ctx.step()                                 # judged on the same state
+        st = ctx.learned_status()
+        e, sc = ctx.learned_history()
+        expect = min(10.0, max(0.1, dd / dl))
+        assert len(e) == 1 and sc[0] == 1.0
+        assert abs(e[0] - err) < 1e-6 * max(err, 1e-6)
+        assert abs(st["scale"] - expect) < 1e-6 * expect
+        ctx.learned_set_scale(2.0, adapt=False)
+        ctx.step()
+        assert ctx.learned_status()["scale"] == 2.0
+        with pytest.raises(fsl.FslError):
+            ctx.learned_set_scale(20.0)
