@@ -538,7 +538,10 @@ int fsl_dynamic_smagorinsky(fsl_ctx* c)
 {
     int rc = ensure_sgs(c);
     if (rc) return rc;
-    launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c, nullptr);
+    if (c->cfg.precision == FSL_PREC_MIXED)
+        launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, nullptr);
+    else
+        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c, nullptr);
     return check_launch(c, "dynamic_smagorinsky");
 }
 

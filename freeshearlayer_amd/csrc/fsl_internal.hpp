@@ -75,6 +75,10 @@ struct SgsBufs {
 size_t sgs_scratch_arrays();   // number of cell arrays SgsBufs needs
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
                 void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
+// the same model in FP32 (filters, Germano contraction, scratch) on the FP64
+// scratch allocation; mu_SGS stays an FP64 array
+void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
+                    void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
 
 // --- learned closure (fsl_mlp.hip) ---
 int mlp_nparam();

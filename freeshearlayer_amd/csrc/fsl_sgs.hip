@@ -140,7 +140,8 @@ __device__ __forceinline__ Tile tile_of(const Geo& g)
 __device__ __forceinline__ int bidx(int bi, int bj, int bk) { return (bi * BJ + bj) * BK + bk; }
 
 // field a of {u, v, w, uu, uv, uw, vv, vw, ww} at box cell b
-__device__ __forceinline__ double field9(const double (*s)[NB], int a, int b)
+template <class T>
+__device__ __forceinline__ T field9(const T (*s)[NB], int a, int b)
 {
     switch (a) {
     case 0: return s[0][b];
@@ -158,16 +159,16 @@ __device__ __forceinline__ double field9(const double (*s)[NB], int a, int b)
 // The three passes of the box filter for one field whose box values come
 // from src(b).  out[q] receives the filtered value at this thread's tile
 // cells; t1i0 (optional) the y-pass value on the i = 0 plane.
-template <class Src>
-__device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, double h, Src src, double* s_t1,
-                                           double* s_t2, double out[CPT], double* t1i0_out, int tid)
+template <class R, class Src>
+__device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, R h, Src src, R* s_t1, R* s_t2,
+                                           R out[CPT], R* t1i0_out, int tid, R t1i0_add = 0)
 {
     for (int e = tid; e < NT1; e += kBlock) {
         const int bk = e % BK, tj = (e / BK) % TJ, bi = e / (BK * TJ);
         const int gi = T.i0 - 1 + bi, gj = T.j0 + tj, gk = T.k0 - 1 + bk;
-        double r = 0.;
+        R r = 0;
         if (gi <= g.L && gj <= g.H && gk <= g.D) {
-            double tmp = 0.;
+            R tmp = 0;
             tmp += src(bidx(bi, tj, bk)) * h;
             tmp += src(bidx(bi, tj + 1, bk)) * h;
             tmp += src(bidx(bi, tj + 2, bk)) * h;
@@ -179,10 +180,10 @@ __device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, double h
     for (int e = tid; e < NT2; e += kBlock) {
         const int tk = e % TK, tj = (e / TK) % TJ, bi = e / (TK * TJ);
         const int gi = T.i0 - 1 + bi, gj = T.j0 + tj, gk = T.k0 + tk;
-        double r = 0.;
+        R r = 0;
         if (gi <= g.L && gj <= g.H && gk <= g.D) {
             const int b1 = (bi * TJ + tj) * BK + tk;
-            double tmp = 0.;
+            R tmp = 0;
             tmp += s_t1[b1] * h;
             tmp += s_t1[b1 + 1] * h;
             tmp += s_t1[b1 + 2] * h;
@@ -196,7 +197,7 @@ __device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, double h
         const int e = tid + q * kBlock;
         const int tk = e % TK, tj = (e / TK) % TJ, ti = e / (TK * TJ);
         const int b2 = (ti * TJ + tj) * TK + tk;
-        double tmp = 0.;
+        R tmp = 0;
         tmp += s_t2[b2] * h;
         tmp += s_t2[b2 + TJ * TK] * h;
         tmp += s_t2[b2 + 2 * TJ * TK] * h;
@@ -207,78 +208,117 @@ __device__ __forceinline__ void box_filter(const Geo& g, const Tile& T, double h
         for (int e = tid; e < TJ * TK; e += kBlock) {
             const int tk = e % TK, tj = e / TK;
             const int gj = T.j0 + tj, gk = T.k0 + tk;
-            if (gj <= g.H && gk <= g.D)
-                t1i0_out[(long long)gj * g.sj + gk] = s_t1[(0 * TJ + tj) * BK + tk + 1];
+            if (gj <= g.H && gk <= g.D) {
+                if (sizeof(R) == sizeof(double)) t1i0_out[(long long)gj * g.sj + gk] = s_t1[(0 * TJ + tj) * BK + tk + 1];
+                else t1i0_out[(long long)gj * g.sj + gk] = s_t1[(0 * TJ + tj) * BK + tk + 1] + t1i0_add;
+            }
         }
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ void strain_box(const double (*s)[NB], int b, const Phys& ph, double S[9])
+template <class R>
+__device__ __forceinline__ void strain_box(const R (*s)[NB], int b, const Phys& ph, R S[9])
 {
-    double gr[3][3];
+    R gr[3][3];
     constexpr int sI = BJ * BK, sJ = BK;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        gr[r][0] = (s[r][b + sI] - s[r][b - sI]) * ph.r2del[0];
-        gr[r][1] = (s[r][b + sJ] - s[r][b - sJ]) * ph.r2del[1];
-        gr[r][2] = (s[r][b + 1] - s[r][b - 1]) * ph.r2del[2];
+        gr[r][0] = (s[r][b + sI] - s[r][b - sI]) * (R)ph.r2del[0];
+        gr[r][1] = (s[r][b + sJ] - s[r][b - sJ]) * (R)ph.r2del[1];
+        gr[r][2] = (s[r][b + 1] - s[r][b - 1]) * (R)ph.r2del[2];
     }
     S[0] = gr[0][0];
-    S[1] = 0.5 * (gr[0][1] + gr[1][0]);
-    S[2] = 0.5 * (gr[0][2] + gr[2][0]);
+    S[1] = (R)0.5 * (gr[0][1] + gr[1][0]);
+    S[2] = (R)0.5 * (gr[0][2] + gr[2][0]);
     S[3] = S[1];
     S[4] = gr[1][1];
-    S[5] = 0.5 * (gr[1][2] + gr[2][1]);
+    S[5] = (R)0.5 * (gr[1][2] + gr[2][1]);
     S[6] = S[2];
     S[7] = S[5];
     S[8] = gr[2][2];
 }
 
-__device__ __forceinline__ double mag9(const double S[9])
+// FP64: correctly rounded sqrt, 1/x and x/y (the reference's operations);
+// FP32: the hardware approximations (v_sqrt_f32, v_rcp_f32, about 1 ulp)
+__device__ __forceinline__ double sqrt_r(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_r(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double rcp_r(double x) { return 1. / x; }
+__device__ __forceinline__ float rcp_r(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double div_r(double a, double b) { return a / b; }
+__device__ __forceinline__ float div_r(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+template <class R>
+__device__ __forceinline__ R mag9(const R S[9])
 {
-    return sqrt(2 * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + S[3] * S[3] + S[4] * S[4] + S[5] * S[5] +
-                     S[6] * S[6] + S[7] * S[7] + S[8] * S[8]));
+    return sqrt_r((R)2 * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + S[3] * S[3] + S[4] * S[4] + S[5] * S[5] +
+                          S[6] * S[6] + S[7] * S[7] + S[8] * S[8]));
 }
 
+template <class R>
 struct SgsA {
     CState5 U;
-    double* uf[3];
-    double* Lij[6];
-    double* Aij[6];
-    double* mag;
+    R* uf[3];
+    R* Lij[6];
+    R* Aij[6];
+    R* mag;
 };
 
-__global__ void __launch_bounds__(kBlock) k_sgs_a(SgsA a, Geo g, Phys ph)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_sgs_a(SgsA<R> a, Geo g, Phys ph)
 {
-    __shared__ double s_u[3][NB];
-    __shared__ double s_t1[NT1];
-    __shared__ double s_t2[NT2];
+    __shared__ R s_u[3][NB];
+    __shared__ R s_t1[NT1];
+    __shared__ R s_t2[NT2];
     const Tile T = tile_of(g);
     const int tid = threadIdx.x;
     // u, v, w = (rho u, rho v, rho w) * (1 / rho) on the box (src/turbulence.c:197-209)
     for (int e = tid; e < NB; e += kBlock) {
         const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
         const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
-        double u = 0., v = 0., w = 0.;
+        R u = 0, v = 0, w = 0;
         if (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) {
             const long long c = cidx(g, gi, gj, gk);
-            const double rr = 1. / a.U.f[0][c];
-            u = a.U.f[1][c] * rr;
-            v = a.U.f[2][c] * rr;
-            w = a.U.f[3][c] * rr;
+            const R rr = rcp_r((R)a.U.f[0][c]);
+            u = (R)a.U.f[1][c] * rr;
+            v = (R)a.U.f[2][c] * rr;
+            w = (R)a.U.f[3][c] * rr;
         }
         s_u[0][e] = u;
         s_u[1][e] = v;
         s_u[2][e] = w;
     }
     __syncthreads();
-    const double h = 1. / 3.;
-    double fil[9][CPT];
+    const R h = (R)(1. / 3.);
+    // FP32: the box's velocities are shifted by the tile-centre velocity a,
+    // u = u' + a, so the products lose no digits to |u| >> |u'|.  With W the
+    // filter's weight sum at a cell (the three passes applied to a field of
+    // ones: about 1 where every window is complete, 2/3 on the planes k = D
+    // and i = L, whose z and x windows the truncated passes cut),
+    //   u^ = f(u') + a W
+    //   L_ij = f(u'_i u'_j) - f(u'_i) f(u'_j) + (1 - W)(a_i f(u'_j) + a_j f(u'_i)) + a_i a_j (W - W^2),
+    // the same values as f(u_i u_j) - u^_i u^_j.  The strain is a difference
+    // and does not see the shift.
+    R sh[3] = {0, 0, 0}, w1 = 0, w2f = 0, w2t = 0;
+    if (sizeof(R) == sizeof(float)) {
+        const int bc = bidx(1 + TI / 2, 1 + TJ / 2, 1 + TK / 2);
+#pragma unroll
+        for (int q = 0; q < 3; q++) sh[q] = s_u[q][bc];
+        __syncthreads();
+        for (int e = tid; e < NB; e += kBlock) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) s_u[q][e] -= sh[q];
+        }
+        __syncthreads();
+        w1 = (((R)0 + h) + h) + h;                       // y pass: always three valid rows
+        w2f = (((R)0 + w1 * h) + w1 * h) + w1 * h;       // z pass, k < D
+        w2t = ((R)0 + w1 * h) + w1 * h;                  // z pass, k = D (plane D + 1 of the y pass is 0)
+    }
+    R fil[9][CPT];
 #pragma unroll
     for (int f = 0; f < 9; f++) {
-        box_filter(g, T, h, [&](int b) { return field9(s_u, f, b); }, s_t1, s_t2, fil[f],
-                   f < 3 ? a.uf[f] : nullptr, tid);
+        box_filter<R>(g, T, h, [&](int b) { return field9(s_u, f, b); }, s_t1, s_t2, fil[f],
+                      f < 3 ? a.uf[f] : nullptr, tid, f < 3 ? sh[f] * w1 : (R)0);
     }
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
@@ -287,19 +327,34 @@ __global__ void __launch_bounds__(kBlock) k_sgs_a(SgsA a, Geo g, Phys ph)
         const int gi = T.i0 + ti, gj = T.j0 + tj, gk = T.k0 + tk;
         if (gi > g.L || gj > g.H || gk > g.D) continue;
         const long long c = cidx(g, gi, gj, gk);
-        const double uf = fil[0][q], vf = fil[1][q], wf = fil[2][q];
-        a.uf[0][c] = uf;
-        a.uf[1][c] = vf;
-        a.uf[2][c] = wf;
-        a.Lij[0][c] = fil[3][q] - uf * uf;
-        a.Lij[1][c] = fil[4][q] - uf * vf;
-        a.Lij[2][c] = fil[5][q] - uf * wf;
-        a.Lij[3][c] = fil[6][q] - vf * vf;
-        a.Lij[4][c] = fil[7][q] - vf * wf;
-        a.Lij[5][c] = fil[8][q] - wf * wf;
-        double S[9];
+        const R uf = fil[0][q], vf = fil[1][q], wf = fil[2][q];
+        if (sizeof(R) == sizeof(double)) {
+            a.uf[0][c] = uf;
+            a.uf[1][c] = vf;
+            a.uf[2][c] = wf;
+            a.Lij[0][c] = fil[3][q] - uf * uf;
+            a.Lij[1][c] = fil[4][q] - uf * vf;
+            a.Lij[2][c] = fil[5][q] - uf * wf;
+            a.Lij[3][c] = fil[6][q] - vf * vf;
+            a.Lij[4][c] = fil[7][q] - vf * wf;
+            a.Lij[5][c] = fil[8][q] - wf * wf;
+        } else {
+            const R w2 = gk < g.D ? w2f : w2t;                                  // t2 on planes i-1, i, i+1
+            const R W = (((R)0 + w2 * h) + w2 * h) + (gi < g.L ? w2 : (R)0) * h;   // plane L + 1 of t2 is 0
+            const R om = (R)1 - W, ww = W - W * W;
+            const R fu[3] = {uf, vf, wf};
+#pragma unroll
+            for (int r = 0; r < 3; r++) a.uf[r][c] = fu[r] + sh[r] * W;
+            const int pr[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                const int x = pr[m][0], y = pr[m][1];
+                a.Lij[m][c] = fil[3 + m][q] - fu[x] * fu[y] + om * (sh[x] * fu[y] + sh[y] * fu[x]) + sh[x] * sh[y] * ww;
+            }
+        }
+        R S[9];
         strain_box(s_u, bidx(ti + 1, tj + 1, tk + 1), ph, S);
-        const double Sm = mag9(S);
+        const R Sm = mag9(S);
         a.mag[c] = Sm;
         const int uniq[6] = {0, 1, 2, 4, 5, 8};   // 11, 12, 13, 22, 23, 33
 #pragma unroll
@@ -307,28 +362,30 @@ __global__ void __launch_bounds__(kBlock) k_sgs_a(SgsA a, Geo g, Phys ph)
     }
 }
 
+template <class R>
 struct SgsB {
     const double* rho;
-    const double* uf[3];
-    const double* Lij[6];
-    const double* Aij[6];
-    const double* mag;
+    const R* uf[3];
+    const R* Lij[6];
+    const R* Aij[6];
+    const R* mag;
     double* mu;
     float* cd;   // optional: the clipped coefficient (training targets)
 };
 
-__global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB<R> a, Geo g, Phys ph)
 {
-    __shared__ double s_uf[3][NB];
-    __shared__ double s_a[NB];
-    __shared__ double s_t1[NT1];
-    __shared__ double s_t2[NT2];
+    __shared__ R s_uf[3][NB];
+    __shared__ R s_a[NB];
+    __shared__ R s_t1[NT1];
+    __shared__ R s_t2[NT2];
     const Tile T = tile_of(g);
     const int tid = threadIdx.x;
     for (int e = tid; e < NB; e += kBlock) {
         const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
         const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
-        double u = 0., v = 0., w = 0.;
+        R u = 0, v = 0, w = 0;
         if (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) {
             const long long c = cidx(g, gi, gj, gk);
             u = a.uf[0][c];
@@ -339,18 +396,18 @@ __global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
         s_uf[1][e] = v;
         s_uf[2][e] = w;
     }
-    const double h = 1. / 3.;
-    double Af[6][CPT];
+    const R h = (R)(1. / 3.);
+    R Af[6][CPT];
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         __syncthreads();   // previous users of s_a are done
         for (int e = tid; e < NB; e += kBlock) {
             const int bk = e % BK, bj = (e / BK) % BJ, bi = e / (BK * BJ);
             const int gi = T.i0 - 1 + bi, gj = T.j0 - 1 + bj, gk = T.k0 - 1 + bk;
-            s_a[e] = (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) ? a.Aij[m][cidx(g, gi, gj, gk)] : 0.;
+            s_a[e] = (gi <= g.L + 1 && gj <= g.H + 1 && gk <= g.D + 1) ? a.Aij[m][cidx(g, gi, gj, gk)] : (R)0;
         }
         __syncthreads();
-        box_filter(g, T, h, [&](int b) { return s_a[b]; }, s_t1, s_t2, Af[m], nullptr, tid);
+        box_filter<R>(g, T, h, [&](int b) { return s_a[b]; }, s_t1, s_t2, Af[m], nullptr, tid);
     }
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
@@ -359,32 +416,33 @@ __global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB a, Geo g, Phys ph)
         const int gi = T.i0 + ti, gj = T.j0 + tj, gk = T.k0 + tk;
         if (gi > g.L || gj > g.H || gk > g.D) continue;
         const long long c = cidx(g, gi, gj, gk);
-        double Sf[9];
+        R Sf[9];
         strain_box(s_uf, bidx(ti + 1, tj + 1, tk + 1), ph, Sf);
-        const double Sfm = mag9(Sf);
+        const R Sfm = mag9(Sf);
         // 3x3 from the six distinct components: 11 12 13 / 21=12 22 23 / 31=13 32=23 33
         const int map9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
         const int uniq[6] = {0, 1, 2, 4, 5, 8};
-        double B6[6], M[9], Lm[9];
+        R B6[6], M[9], Lm[9];
 #pragma unroll
         for (int m = 0; m < 6; m++) B6[m] = Sfm * Sf[uniq[m]];
 #pragma unroll
         for (int r = 0; r < 9; r++) {
-            M[r] = ph.DD * (4 * B6[map9[r]] - Af[map9[r]][q]);
+            M[r] = (R)ph.DD * ((R)4 * B6[map9[r]] - Af[map9[r]][q]);
             Lm[r] = a.Lij[map9[r]][c];
         }
-        const double tr = -(1.0 / 3.0) * (Lm[0] + Lm[4] + Lm[8]);
+        const R tr = -(R)(1.0 / 3.0) * (Lm[0] + Lm[4] + Lm[8]);
         Lm[0] += tr;
         Lm[4] += tr;
         Lm[8] += tr;
-        const double LLMM = (Lm[0] * M[0] + Lm[1] * M[1] + Lm[2] * M[2] + Lm[3] * M[3] + Lm[4] * M[4] +
+        const R LLMM = (Lm[0] * M[0] + Lm[1] * M[1] + Lm[2] * M[2] + Lm[3] * M[3] + Lm[4] * M[4] +
                              Lm[5] * M[5] + Lm[6] * M[6] + Lm[7] * M[7] + Lm[8] * M[8]);
-        const double MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
+        const R MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
                              M[5] * M[5] + M[6] * M[6] + M[7] * M[7] + M[8] * M[8]);
-        double Cd = -0.5 * (LLMM / (MMMM + kSmall));
-        if (Cd > 0.15) Cd = 0.15;
-        else if (Cd < 0.0) Cd = 0.0;
-        a.mu[c] = a.rho[c] * Cd * ph.DD * a.mag[c];
+        R Cd = -(R)0.5 * div_r(LLMM, MMMM + (R)kSmall);
+        if (Cd > (R)0.15) Cd = (R)0.15;
+        else if (Cd < (R)0.0) Cd = (R)0.0;
+        if (sizeof(R) == sizeof(double)) a.mu[c] = a.rho[c] * (double)Cd * ph.DD * (double)a.mag[c];
+        else a.mu[c] = (double)((R)a.rho[c] * Cd * (R)ph.DD * a.mag[c]);
         if (a.cd) a.cd[c] = (float)Cd;
     }
 }
@@ -426,34 +484,57 @@ void launch_filter(hipStream_t s, const double* in, double* out, double* t1, dou
     run_filters(s, 1, &in, &out, &t1, &t2, g, h, prof, pa);
 }
 
-void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
-                void (*prof)(void*, int, int), void* pa, float* cd_out)
+namespace {
+
+template <class R>
+void sgs_pair(hipStream_t s, CState5 U, double* mu, R* const uf[3], R* const Lij[6], R* const Aij[6], R* mag,
+              const Geo& g, const Phys& ph, void (*prof)(void*, int, int), void* pa, float* cd_out)
 {
     const int ntiles = ((g.L + TI - 1) / TI) * ((g.H + TJ - 1) / TJ) * ((g.D + TK - 1) / TK);
-    SgsA A{};
+    SgsA<R> A{};
     A.U = U;
-    for (int r = 0; r < 3; r++) A.uf[r] = B.uf[r];
+    for (int r = 0; r < 3; r++) A.uf[r] = uf[r];
     for (int m = 0; m < 6; m++) {
-        A.Lij[m] = B.Lij[m];
-        A.Aij[m] = B.Aij[m];
+        A.Lij[m] = Lij[m];
+        A.Aij[m] = Aij[m];
     }
-    A.mag = B.mag;
+    A.mag = mag;
     if (prof) prof(pa, K_SGS_PRIM, 0);
-    hipLaunchKernelGGL(k_sgs_a, dim3(ntiles), dim3(kBlock), 0, s, A, g, ph);
+    hipLaunchKernelGGL(k_sgs_a<R>, dim3(ntiles), dim3(kBlock), 0, s, A, g, ph);
     if (prof) prof(pa, K_SGS_PRIM, 1);
-    SgsB Bk{};
+    SgsB<R> Bk{};
     Bk.rho = U.f[0];
-    for (int r = 0; r < 3; r++) Bk.uf[r] = B.uf[r];
+    for (int r = 0; r < 3; r++) Bk.uf[r] = uf[r];
     for (int m = 0; m < 6; m++) {
-        Bk.Lij[m] = B.Lij[m];
-        Bk.Aij[m] = B.Aij[m];
+        Bk.Lij[m] = Lij[m];
+        Bk.Aij[m] = Aij[m];
     }
-    Bk.mag = B.mag;
+    Bk.mag = mag;
     Bk.mu = mu;
     Bk.cd = cd_out;
     if (prof) prof(pa, K_SGS_FINAL, 0);
-    hipLaunchKernelGGL(k_sgs_b, dim3(ntiles), dim3(kBlock), 0, s, Bk, g, ph);
+    hipLaunchKernelGGL(k_sgs_b<R>, dim3(ntiles), dim3(kBlock), 0, s, Bk, g, ph);
     if (prof) prof(pa, K_SGS_FINAL, 1);
+}
+
+}  // namespace
+
+void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
+                void (*prof)(void*, int, int), void* pa, float* cd_out)
+{
+    sgs_pair<double>(s, U, mu, B.uf, B.Lij, B.Aij, B.mag, g, ph, prof, pa, cd_out);
+}
+
+void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
+                    void (*prof)(void*, int, int), void* pa, float* cd_out)
+{
+    // the FP64 scratch holds 16 cell arrays; the FP32 form uses 16 float arrays in its first half
+    float* p = (float*)scratch;
+    float *uf[3], *Lij[6], *Aij[6];
+    for (int r = 0; r < 3; r++) { uf[r] = p; p += g.G; }
+    for (int m = 0; m < 6; m++) { Lij[m] = p; p += g.G; }
+    for (int m = 0; m < 6; m++) { Aij[m] = p; p += g.G; }
+    sgs_pair<float>(s, U, mu, uf, Lij, Aij, p, g, ph, prof, pa, cd_out);
 }
 
 }  // namespace fsl

@@ -143,7 +143,7 @@ typedef struct fsl_ctx fsl_ctx;
 #define FSL_SGS_LEARNED  2   /* per-cell MLP for Cs^2, dynamic model as judge */
 
 #define FSL_PREC_FP64    0   /* parity path: FP64, no contraction, reference op order */
-#define FSL_PREC_MIXED   1   /* FP32 stencils, FP64 state / RK / p recovery / CFL */
+#define FSL_PREC_MIXED   1   /* FP32 stencils and SGS, FP64 state / RK / p recovery / CFL */
 
 #define FSL_CFL_CONVECTIVE 0 /* src/helpers.c:23-52: max (rho u / rho) dt/dx, rescale */
 #define FSL_CFL_ACOUSTIC   1 /* dt = cfl / max[(|u|+c)/dx + (|v|+c)/dy + (|w|+c)/dz] */
@@ -218,7 +218,9 @@ int fsl_rk_update(fsl_ctx *ctx, int scheme, int stage);
 /* src/helpers.c:23-52: device max-reduction, then deltaT *= maxCoNum / maxCo
    (convective) or deltaT = cfl / max(...) (acoustic).  maxCo may be NULL. */
 int fsl_check_co_num(fsl_ctx *ctx, double *maxCo);
-/* src/turbulence.c:103-507 on the step state U -> mu_SGS */
+/* src/turbulence.c:103-507 on the step state U -> mu_SGS (FP64 contexts:
+   the reference's operations; mixed-precision contexts: the same model with
+   FP32 filters, contraction and scratch) */
 int fsl_dynamic_smagorinsky(fsl_ctx *ctx);
 /* src/turbulence.c:17-100 with kernel h (the model uses {1/3,1/3,1/3}):
    host cell array in -> host cell array out (the returned array of the

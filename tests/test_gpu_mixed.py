@@ -58,3 +58,44 @@ def test_mixed_reference_case_runs_and_stays_close():
     e = rel_err(a, b)
     print("reference case, 10 steps:", ["%.2e" % x for x in e])
     assert max(e) < 1e-3
+
+
+def test_fp32_dynamic_model_vs_fp64():
+    """The mixed path's dynamic model (FP32 filters, Germano contraction and
+    scratch, velocities shifted per tile so the Leonard stress does not
+    cancel) against the FP64 one on the same developed state: bounds on the
+    bulk and the worst cell relative to the largest mu_SGS, and on the total
+    (the dissipation level the closure sets)."""
+    case = fsl.case_reference(48, 32, 24)
+    U, u_in, mu = fsl.ic_shear_layer(case)
+    case.mu_L = mu
+    cfg64 = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC, fused=1, cfl_mode=fsl.FSL_CFL_ACOUSTIC, cfl=0.5)
+    with fsl.Context(case, cfg64) as ctx:
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        ctx.set_lcg(0.317391239817, 0)
+        ctx.run(60)
+        S = ctx.download_state()
+        ctx.ghost_fill()
+        ctx.dynamic_smagorinsky()
+        m64 = ctx.download_mu()
+    cfg32 = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC, fused=1, precision=fsl.FSL_PREC_MIXED)
+    with fsl.Context(case, cfg32) as ctx:
+        ctx.set_inflow(u_in)
+        ctx.upload_state(S)
+        ctx.ghost_fill()
+        ctx.dynamic_smagorinsky()
+        m32 = ctx.download_mu()
+    I = (slice(1, -1),) * 3
+    a, b = m32[I], m64[I]
+    scale = np.abs(b).max()
+    d = np.abs(a - b) / scale
+    print("FP32 vs FP64 dynamic mu_SGS: median %.2e, p99 %.2e, max %.2e of max mu; dissipation ratio %.6f"
+          % (np.median(d), np.percentile(d, 99), d.max(), a.sum() / b.sum()))
+    assert np.all(np.isfinite(a)) and scale > 0
+    assert np.median(d) < 1e-7 and np.percentile(d, 99) < 1e-5 and d.max() < 1e-4
+    assert abs(a.sum() / b.sum() - 1) < 1e-5
+    # ghost layer untouched
+    gh = np.ones_like(m32, dtype=bool)
+    gh[I] = False
+    assert np.all(m32[gh] == 0)
