@@ -11,11 +11,14 @@ and at the outflow plane), delta_w, the integrated resolved TKE and the
 mass and energy totals; the result gives their relative differences over
 time, the final state's field error, the z energy spectrum of u' on the
 centre plane (with --spectra), and each path's Mcell-updates/s per RK stage
-(timed without the diagnostics).
+(timed without the diagnostics).  With --ref-steps the reference leg stops
+early and the comparison covers its steps (spectra at that step).
 
 Usage:
   python -m freeshearlayer_amd.study learned --mesh 384x256x256 --mach 0.4 --steps 2000 --out r.json
   python -m freeshearlayer_amd.study mixed --mesh 384x256x256 --mach 0.3 --steps 2000 --out r.json
+  python -m freeshearlayer_amd.study learned --mesh 512x384x384 --mach 0.6 --reynolds 2000 --pert 0.03
+      --steps 5000 --ref-steps 1000 --spectra --out r.json
 """
 import argparse
 import json
@@ -74,11 +77,14 @@ def run(path, mesh, args, model):
         rec["energy"].append(float(sums[4]))
         rec["dt"].append(ctx.dt)
 
+    nsteps = args.ref_steps if (path == "reference" and args.ref_steps) else args.steps
     record(0)
     timed = 0.0
     done = 0
-    while done < args.steps:
-        n = min(args.every, args.steps - done)
+    while done < nsteps:
+        n = min(args.every, nsteps - done)
+        if args.spectra and args.ref_steps and done < args.ref_steps < done + n:
+            n = args.ref_steps - done   # stop on the comparison step
         ctx.synchronize()
         t0 = time.perf_counter()
         ctx.run(n)
@@ -86,21 +92,24 @@ def run(path, mesh, args, model):
         timed += time.perf_counter() - t0
         done += n
         record(done)
+        if args.spectra and done == (args.ref_steps or nsteps):
+            rec["spectrum_z"] = spectrum(ctx, case)[0].tolist()
+            rec["spectrum_step"] = done
+        if not (np.isfinite(rec["mass"][-1]) and np.isfinite(rec["energy"][-1])):
+            rec["diverged_at"] = done
+            log("  %s: non-finite state after step %d" % (path, done))
+            break
         log("  %s %s/%s step %d: delta_theta %.4f, tke %.4e, %.2f ms/step" % (
             path, sgs, prec, done, rec["delta_theta"][-1], rec["tke"][-1], 1e3 * timed / done))
     N = L * H * D
-    rec["ms_per_step"] = 1e3 * timed / args.steps
-    rec["mcell_updates_per_s_per_stage"] = N * 3 / (timed / args.steps) / 1e6
+    rec["ms_per_step"] = 1e3 * timed / done
+    rec["mcell_updates_per_s_per_stage"] = N * 3 / (timed / done) / 1e6
     if sgs == "learned":
         rec["learned_status"] = ctx.learned_status()
         e, sc = ctx.learned_history()
         rec["judge_error"] = e.tolist()
         rec["judge_scale"] = sc.tolist()
-    if args.spectra:
-        E, S = spectrum(ctx, case)
-        rec["spectrum_z"] = E.tolist()
-    else:
-        S = ctx.download_state()
+    S = ctx.download_state() if not args.ref_steps else None
     ctx.close()
     return rec, S
 
@@ -120,6 +129,8 @@ def main(argv=None):
     ap.add_argument("--interval", type=int, default=50)
     ap.add_argument("--tol", type=float, default=0.10)
     ap.add_argument("--spectra", action="store_true")
+    ap.add_argument("--ref-steps", type=int, default=0,
+                    help="steps of the reference leg when shorter than the candidate's (compared over those)")
     ap.add_argument("--out")
     a = ap.parse_args(argv)
     if a.mode == "mixed":
@@ -130,15 +141,19 @@ def main(argv=None):
     alt, Sb = run("candidate", mesh, a, a.model)
 
     def rel(k):
-        x, y = np.array(alt[k]), np.array(ref[k])
+        n = min(len(alt[k]), len(ref[k]))   # records at the common steps
+        x, y = np.array(alt[k][:n]), np.array(ref[k][:n])
         return (np.abs(x - y) / np.maximum(np.abs(y), 1e-300)).tolist()
 
     cmp = {k: rel(k) for k in ("delta_theta", "delta_theta_out", "delta_w", "tke", "mass", "energy")}
-    # field error of the final state, per conserved field, against the field's scale
-    I = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
-    A, B = Sa[I], Sb[I]
-    scale = [np.abs(A[0]).max(), np.abs(A[1]).max(), np.abs(A[1]).max(), np.abs(A[1]).max(), np.abs(A[4]).max()]
-    field = [float(np.abs(B[f] - A[f]).max() / scale[f]) for f in range(5)]
+    field = None
+    if Sa is not None and Sb is not None:
+        # field error of the final state, per conserved field, against the field's scale
+        I = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        A, B = Sa[I], Sb[I]
+        scale = [np.abs(A[0]).max(), np.abs(A[1]).max(), np.abs(A[1]).max(), np.abs(A[1]).max(),
+                 np.abs(A[4]).max()]
+        field = [float(np.abs(B[f] - A[f]).max() / scale[f]) for f in range(5)]
     out = {"mode": a.mode, "mesh": a.mesh, "mach": a.mach, "reynolds": a.reynolds, "pert": a.pert,
            "steps": a.steps, "reference": ref, "candidate": alt, "relative_difference": cmp,
            "final_field_error": field,
@@ -146,10 +161,12 @@ def main(argv=None):
     if a.spectra:
         Ea, Eb = np.array(ref["spectrum_z"]), np.array(alt["spectrum_z"])
         out["spectrum_rel_difference"] = (np.abs(Eb - Ea) / np.maximum(Ea, 1e-300)).tolist()
-    log("final: delta_theta %+.2f %%, tke %+.2f %%, mass %.2e, energy %.2e; field error %s; speedup %.2fx "
+    n = min(len(alt["delta_theta"]), len(ref["delta_theta"])) - 1
+    log("at step %d: delta_theta %+.2f %%, tke %+.2f %%, mass %.2e, energy %.2e; field error %s; speedup %.2fx "
         "(%.0f vs %.0f Mcell-updates/s per stage)" % (
-            100 * (alt["delta_theta"][-1] / ref["delta_theta"][-1] - 1), 100 * (alt["tke"][-1] / ref["tke"][-1] - 1),
-            cmp["mass"][-1], cmp["energy"][-1], " ".join("%.1e" % v for v in field), out["speedup"],
+            alt["step"][n], 100 * (alt["delta_theta"][n] / ref["delta_theta"][n] - 1),
+            100 * (alt["tke"][n] / ref["tke"][n] - 1), cmp["mass"][-1], cmp["energy"][-1],
+            " ".join("%.1e" % v for v in field) if field else "-", out["speedup"],
             alt["mcell_updates_per_s_per_stage"], ref["mcell_updates_per_s_per_stage"]))
     log("max over time: delta_theta %.2f %%, tke %.2f %%" % (100 * max(cmp["delta_theta"]), 100 * max(cmp["tke"])))
     if a.out:

@@ -55,3 +55,45 @@ def test_fused_equals_unfused_at_bench_size(quirk):
             out.append(ctx.download_state()[:, 1:-1, 1:-1, 1:-1])
     assert_bitwise(out[0], out[1], "fused vs unfused")
     assert np.all(np.isfinite(out[1]))
+
+
+@pytest.mark.parametrize("dynamic", [0, 1])
+def test_hundred_steps_shear_layer_divergence(dynamic):
+    """The CPU-runnable configuration: the shear layer (tanh profile, Mc 0.3,
+    U2/U1 0.5, Re 500, 2 % seeded perturbations) on 64 x 48 x 32 cells, 3-stage
+    TVD RK, CFL 0.5 (the reference's convective rescale), 100 steps on the
+    fused FP64 path against the oracle.  The
+    divergence (max relative difference per conserved field) is reported at
+    every step and must stay 0: the path is bit-identical, so no drift can
+    accumulate."""
+    from oracle_lib import Oracle
+    from freeshearlayer_amd import shear_case
+    L, H, D = 64, 48, 32
+    case, U, u_in, shear = shear_case.initial(L, H, D)
+    case.maxCoNum = 0.5   # CFL 0.5 through the reference's convective rescale
+    orc = Oracle(L, H, D, dynamic=dynamic)
+    orc.set("maxCoNum", 0.5)
+    orc.set("mu_L", case.mu_L)
+    orc.set("BL_HIG", case.BL_HIG)
+    orc.u_in()[:] = u_in
+    orc.derive()
+    orc.set_state(U)
+    orc.set("X", 0.317391239817)
+    cfg = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC if dynamic else fsl.FSL_SGS_CONSTANT, fused=1)
+    worst = 0.0
+    with fsl.Context(case, cfg) as ctx:
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        ctx.set_lcg(0.317391239817, 0)
+        for n in range(1, 101):
+            ctx.step()
+            orc.step()
+            a, b = ctx.download_state()[:, 1:-1, 1:-1, 1:-1], orc.state()[:, 1:-1, 1:-1, 1:-1]
+            div = [float(np.abs(a[f] - b[f]).max() / np.abs(b[f]).max()) for f in range(5)]
+            worst = max(worst, max(div))
+            if n in (1, 10, 50, 100):
+                print("step %3d: dt %.6e, divergence per field %s" % (n, ctx.dt, " ".join("%.1e" % d for d in div)))
+            assert ctx.dt == orc.get("dt"), "dt at step %d" % n
+            assert max(div) == 0.0, "step %d: %s" % (n, div)
+    orc.close()
+    assert worst == 0.0
