@@ -128,6 +128,10 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
         load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
+        // step state of the cell this iteration updates (i - 1), issued early:
+        // its use after the last barrier would otherwise wait for the load
+        double bq[5];
+        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si + col, bq);
         __syncthreads();   // the previous plane's LDS readers are done
         const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {
@@ -212,8 +216,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                     const double s = w0[f] + lx + ly[f] + lz[f];
                     double o;
                     if (STAGE == 1) o = s;
-                    else if (STAGE == 2) o = 0.75 * a.base.f[f][cc] + 0.25 * s;
-                    else o = (1. / 3.) * a.base.f[f][cc] + (2. / 3.) * s;
+                    else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
+                    else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
                     a.out.f[f][cc] = o;
                 }
             }

@@ -47,6 +47,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     __shared__ double s_zm[5][NT];      // - face states along z
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
+    __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
 
     const Geo& g = a.g;
     const int L = g.L, H = g.H, D = g.D;
@@ -99,6 +100,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
         load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
+        // step state of the cell this iteration updates (i - 1), issued early:
+        // its use after the last barrier would otherwise wait for the load
+        double bq[5];
+        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si + col, bq);
+        // own cell's mu_SGS; the faces take theirs from the LDS ring
+        const double mu_c = a.ph.dynamic ? a.mu[(long long)i * si + col] : 0.;
         __syncthreads();
         const int vc = i & 1, vb = vc ^ 1;
         {
@@ -114,6 +121,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             s_visc[vc][2][c] = vd[2];
             s_visc[vc][3][c] = Tc;
             s_visc[vc][4][c] = (float)w1[0];
+            s_mu[vc][c] = mu_c;
         }
         mx::PrimF P;
         mx::cell_prims(w1, P);
@@ -168,10 +176,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                             s_vr[rb][q][c - 1]) * r4[2];
             }
             float mu_dyn = 0.f;
-            if (a.ph.dynamic) {
-                const long long m = (long long)i * si + col;
-                mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
-            }
+            if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c] + s_mu[vb][c]));
             float Fx[5];
             mx::face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
                              s_visc[vb][4][c], gr, mu_dyn, Fx);
@@ -183,8 +188,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     const double s = w0[f] + lx + (double)ly[f] + (double)lz[f];
                     double o;
                     if (STAGE == 1) o = s;
-                    else if (STAGE == 2) o = 0.75 * a.base.f[f][cc] + 0.25 * s;
-                    else o = (1. / 3.) * a.base.f[f][cc] + (2. / 3.) * s;
+                    else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
+                    else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
                     a.out.f[f][cc] = o;
                 }
             }
@@ -228,11 +233,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
                                 s_vr[rc][q][c - 1]) * r4[2];
                 }
-                float mu_dyn = 0.f;
-                if (a.ph.dynamic) {
-                    const long long m = (long long)i * si + (long long)(jc + 1) * sj + kc;
-                    mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
-                }
+                float mu_dyn = 0.f;   // cell (i, j + 1) and its x neighbour (src/fluxes.c:360)
+                if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + BK] + s_mu[vb][c + BK]));
                 float Fy[5];
                 mx::face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + BK], Tb, s_visc[vc][4][c + BK], Rb,
                                  gr, mu_dyn, Fy);
@@ -254,11 +256,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
                                 s_vr[rc][q][c - BK]) * r4[1];
                 }
-                float mu_dyn = 0.f;
-                if (a.ph.dynamic) {
-                    const long long m = (long long)i * si + (long long)jc * sj + kc + 1;
-                    mu_dyn = (float)(0.5 * (a.mu[m] + a.mu[m - si]));
-                }
+                float mu_dyn = 0.f;   // cell (i, k + 1) and its x neighbour (src/fluxes.c:531)
+                if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + 1] + s_mu[vb][c + 1]));
                 float Fz[5];
                 mx::face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + 1], Tb, s_visc[vc][4][c + 1], Rb, gr,
                                  mu_dyn, Fz);
