@@ -20,7 +20,9 @@ Prints one JSON line.  `value` is the headline path's throughput;
 ratio.  `roofline` prices the dominant kernel's algorithmic bytes per
 launch against its average launch duration, measured with HIP events on the
 context's stream over the timed steps; `mlp_roofline` the learned-closure
-kernel's MFMA FLOPs against the BF16 dense peak.  `cpu_baseline` times the
+kernel's MFMA FLOPs against the BF16 dense peak.  `traffic` (HBM bytes per
+launch) and the VALU / MFMA busy fractions come from rocprofv3 counter runs
+of the same command (profiles/).  `cpu_baseline` times the
 FP64 CPU oracle (the reference algorithm, one host core) on a bounded
 sample -- a reported baseline, not the target.
 
@@ -121,14 +123,19 @@ def roofline_of(prof, L, H, D, sgs, workload):
     if b is None:
         return None, kernels
     avg_s = ms / cnt / 1e3
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("%s@%s" % (name, workload))
+    prof_tab = profile_table()
+    traffic = prof_tab.get("%s@%s" % (name, workload))
     ach = b / avg_s / 1e9
     return ({"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
-             "alg_bytes_per_launch": int(b)}, kernels)
+             "alg_bytes_per_launch": int(b),
+             "valu_busy_pct": prof_tab.get("valu_busy@%s@%s" % (name, workload))}, kernels)
+
+
+def profile_table():
+    """Per-kernel figures measured with rocprofv3 counters (profiles/)."""
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    return json.load(open(tpath)) if os.path.exists(tpath) else {}
 
 
 def cpu_baseline(steps=2, mesh=(192, 128, 96)):
@@ -204,9 +211,12 @@ def main(argv=None):
     if "mlp_mu" in leg["prof"]:
         ms, cnt = leg["prof"]["mlp_mu"]
         tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
+        tab = profile_table()
         mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
-                        "flop_per_cell": MLP_FLOP_PER_CELL, "avg_launch_us": round(ms / cnt * 1e3, 2)}
+                        "flop_per_cell": MLP_FLOP_PER_CELL, "avg_launch_us": round(ms / cnt * 1e3, 2),
+                        "mfma_busy_pct": tab.get("mfma_busy@mlp_mu@%s" % wl),
+                        "valu_busy_pct": tab.get("valu_busy@mlp_mu@%s" % wl)}
     final_dt = ctx.dt
     ctx.close()
     compare = None
