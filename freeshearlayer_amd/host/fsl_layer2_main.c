@@ -8,7 +8,10 @@
  * (src-par/mpi_layer2.c:169-191), and writes backup.dat.
  *
  *   fsl_layer2 [-i layer2.ini | -b mpi_layer2.bin] [-n steps] [--const-cs]
- *              [--fused] [--mixed] [--acoustic CFL] [--real4] [-q] < seed
+ *              [--fused] [--mixed] [--acoustic CFL] [--learned MODEL] [--real4] [-q] < seed
+ *
+ * --learned MODEL runs the learned SGS closure (parameter file as written by
+ * fsl_mlp_file_write), judged against the dynamic model every 50 steps.
  */
 #include "fsl.h"
 
@@ -37,7 +40,7 @@ static int stopfile_says_stop(void)
 
 int main(int argc, char **argv)
 {
-    const char *ini = "layer2.ini", *bin = 0;
+    const char *ini = "layer2.ini", *bin = 0, *model = 0;
     long nsteps = -1;
     int quiet = 0, real_bytes = 8, a, rc;
     double X = 0.0, total = 0.0;
@@ -58,11 +61,14 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "--acoustic") && a + 1 < argc) {
             cfg.cfl_mode = FSL_CFL_ACOUSTIC;
             cfg.cfl = atof(argv[++a]);
+        } else if (!strcmp(argv[a], "--learned") && a + 1 < argc) {
+            cfg.sgs_model = FSL_SGS_LEARNED;
+            model = argv[++a];
         } else if (!strcmp(argv[a], "--real4")) real_bytes = 4;
         else if (!strcmp(argv[a], "-q")) quiet = 1;
         else {
             fprintf(stderr, "usage: %s [-i ini | -b bin] [-n steps] [--const-cs] [--fused] [--mixed] "
-                            "[--acoustic CFL] [--real4] [-q] < seed\n", argv[0]);
+                            "[--acoustic CFL] [--learned MODEL] [--real4] [-q] < seed\n", argv[0]);
             return 2;
         }
     }
@@ -104,6 +110,16 @@ int main(int argc, char **argv)
         fsl_set_dt(ctx, dt);
         fsl_set_step(ctx, step0);
     }
+    if (model) {
+        static float params[FSL_MLP_NPARAM];
+        float norm[16];
+        if (fsl_mlp_file_read(model, params, norm) != FSL_OK || fsl_mlp_set_params(ctx, params, norm) != FSL_OK) {
+            fprintf(stderr, "cannot load the learned closure from %s\n", model);
+            fsl_destroy(ctx);
+            free(U);
+            return 1;
+        }
+    }
     fsl_set_lcg(ctx, X, 0);
     rc = fsl_upload_state(ctx, 0, U);
     while (rc == FSL_OK && cont_flag && (long)fsl_get_step(ctx) < nsteps) {
@@ -113,6 +129,12 @@ int main(int argc, char **argv)
         if (stopfile_says_stop()) break;
     }
     if (rc != FSL_OK) fprintf(stderr, "step failed: %s\n", fsl_last_error(ctx));
+    if (model && !quiet) {
+        fsl_learned_status st;
+        fsl_learned_get_status(ctx, &st);
+        printf("Learned closure: %lld steps on the network, %lld on the dynamic model, %lld judges, "
+               "%lld fallbacks\n", st.learned_steps, st.dynamic_steps, st.judged, st.fallbacks);
+    }
     if (fsl_download_state(ctx, 0, U) == FSL_OK)
         fsl_backup_write("backup.dat", &c, U, fsl_get_dt(ctx), fsl_get_step(ctx), real_bytes);
     fsl_destroy(ctx);
