@@ -6,7 +6,8 @@
 //   * the conserved state and every difference taken from it (the
 //     reconstruction's forward/backward differences are formed in FP64 and
 //     rounded once),
-//   * the face states (cell value in FP64 plus an FP32 correction),
+//   * the face energy (cell value in FP64 plus an FP32 correction); face
+//     density and momenta are FP32 (FaceF),
 //   * pressure recovered from total energy, p = (K-1)(E - rho|u|^2/2): E is
 //     FP64 and only the kinetic part is FP32, so the cancellation at low Mach
 //     costs no FP32 digits,
@@ -26,6 +27,28 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 
 // pressure from FP64 total energy and an FP32 kinetic energy
 __device__ __forceinline__ float pressure(double E, float ke) { return (float)((E - (double)ke) * kK_1); }
+
+// face state: FP32 density and momenta, FP64 total energy (pressure recovery)
+struct FaceF {
+    float f[4];
+    double E;
+};
+
+__device__ __forceinline__ FaceF to_face(const double q[5])
+{
+    FaceF o;
+#pragma unroll
+    for (int r = 0; r < 4; r++) o.f[r] = (float)q[r];
+    o.E = q[4];
+    return o;
+}
+
+// outflow: momenta kept, E rebuilt with p_out (src/bounCondOnInterfaces.c:63-71)
+__device__ __forceinline__ void outflow_face(double p_out, FaceF& q)
+{
+    const double R = q.f[0], u = q.f[1], v = q.f[2], w = q.f[3];
+    q.E = p_out / kK_1 + 0.5 * (u * u + v * v + w * w) / R;
+}
 
 struct PrimF {
     float R, vel[3], P, C, aa, ll, ee, gg, hh, nn, S11;
@@ -59,8 +82,8 @@ __device__ __forceinline__ void cell_prims(const double q[5], PrimF& c)
 }
 
 template <int N, int T1, int T2>
-__device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], const double qp[5],
-                                          const double qm[5], bool quirk, double plus[5], double minus[5])
+__device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], const float qf[4], const double qp[5],
+                                          const double qm[5], bool quirk, FaceF& plus, FaceF& minus)
 {
 #pragma clang fp contract(fast)
     const int perm[5] = {0, N, T1, T2, 4};
@@ -110,7 +133,8 @@ __device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], con
     d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
     d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
 #pragma unroll
-    for (int r = 0; r < 5; r++) plus[r] = q[r] + (double)d[r];
+    for (int r = 0; r < 4; r++) plus.f[r] = qf[r] + d[r];
+    plus.E = q[4] + (double)d[4];
 #pragma unroll
     for (int r = 0; r < 5; r++) kv[r] = Mh[r] * C2f + Wh[r] * C1f;
     kk = kv[3] + kv[4];
@@ -120,20 +144,21 @@ __device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], con
     d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
     d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
 #pragma unroll
-    for (int r = 0; r < 5; r++) minus[r] = q[r] - (double)d[r];
+    for (int r = 0; r < 4; r++) minus.f[r] = qf[r] - d[r];
+    minus.E = q[4] - (double)d[4];
 }
 
 template <int N>
-__device__ __forceinline__ void riemann(const double l[5], const double r[5], float& Ro, float vel[3], float& Po)
+__device__ __forceinline__ void riemann(const FaceF& l, const FaceF& r, float& Ro, float vel[3], float& Po)
 {
 #pragma clang fp contract(fast)
     float lv[3], rv[3];
-    const float lR = (float)l[0], rR = (float)r[0];
+    const float lR = l.f[0], rR = r.f[0];
     const float lRi = rcp(lR), rRi = rcp(rR);
-    lv[0] = (float)l[1] * lRi; lv[1] = (float)l[2] * lRi; lv[2] = (float)l[3] * lRi;
-    rv[0] = (float)r[1] * rRi; rv[1] = (float)r[2] * rRi; rv[2] = (float)r[3] * rRi;
-    const float lP = pressure(l[4], 0.5f * lR * (lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]));
-    const float rP = pressure(r[4], 0.5f * rR * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]));
+    lv[0] = l.f[1] * lRi; lv[1] = l.f[2] * lRi; lv[2] = l.f[3] * lRi;
+    rv[0] = r.f[1] * rRi; rv[1] = r.f[2] * rRi; rv[2] = r.f[3] * rRi;
+    const float lP = pressure(l.E, 0.5f * lR * (lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]));
+    const float rP = pressure(r.E, 0.5f * rR * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]));
     const float lC = fsqrt(1.4f * lP * lRi), rC = fsqrt(1.4f * rP * rRi);
     const float ljo = -0.4f * lP, rjo = -0.4f * rP;
     const float lRU = lP * rcp(lR * lC), rRU = rP * rcp(rR * rC);

@@ -312,6 +312,47 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
     }
 }
 
+// Segments along x.  Workgroups of one launch run in rounds of `resident`
+// (per_cu per CU); a round takes about (segment length + 2) plane steps
+// (the march starts two planes early).  Pick the count that minimises
+// rounds x (length + 2), i.e. long segments without a mostly idle last round.
+int device_cus()
+{
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+            n = 256;
+    }
+    return n;
+}
+
+}  // namespace
+
+int fused_segments(int L, int cols, int per_cu, int* seg_len)
+{
+    const long long resident = (long long)per_cu * device_cus();
+    int best = 1, best_len = L;
+    long long best_cost = -1;
+    for (int nseg = 1; nseg <= L; nseg++) {
+        const int len = (L + nseg - 1) / nseg;
+        if (len < 4 && L >= 4) break;
+        if ((L + len - 1) / len != nseg) continue;   // same length as a smaller count
+        const long long rounds = ((long long)nseg * cols + resident - 1) / resident;
+        const long long cost = rounds * (len + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = nseg;
+            best_len = len;
+        }
+    }
+    *seg_len = best_len;
+    return best;
+}
+
+namespace {
+
 struct Launch {
     dim3 grid, block;
     int segL, nseg, ntj, ntk, nblocks, remap;
@@ -324,14 +365,8 @@ Launch plan(const Geo& g, int BJ, int BK, int per_cu)
     p.ntj = (g.H + TJ - 1) / TJ;
     p.ntk = (g.D + TK - 1) / TK;
     const int cols = p.ntj * p.ntk;
-    // about four resident rounds of workgroups over 256 CUs
-    const int target = 4 * 256 * per_cu;
-    int nseg = (target + cols - 1) / cols;
-    if (nseg > g.L) nseg = g.L;
-    if (nseg < 1) nseg = 1;
-    int segL = (g.L + nseg - 1) / nseg;
-    if (segL < 4 && g.L >= 4) segL = 4;
-    nseg = (g.L + segL - 1) / segL;
+    int segL = 0;
+    const int nseg = fused_segments(g.L, cols, per_cu, &segL);
     p.segL = segL;
     p.nseg = nseg;
     p.nblocks = nseg * cols;

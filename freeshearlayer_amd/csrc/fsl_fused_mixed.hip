@@ -43,8 +43,10 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
     __shared__ float s_vr[3][3][NT];    // 1/rho velocities, planes i-1, i, i+1
     __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
-    __shared__ double s_ym[5][NT];      // - face states along y
-    __shared__ double s_zm[5][NT];      // - face states along z
+    __shared__ float s_ym[4][NT];       // - face states along y: density, momenta
+    __shared__ double s_ymE[NT];        //   and energy
+    __shared__ float s_zm[4][NT];       // - face states along z
+    __shared__ double s_zmE[NT];
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
     __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
@@ -81,10 +83,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
 
     double w0[5], w1[5], w2[5];
-    double xp_prev[5];
+    mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
 #pragma unroll
-    for (int f = 0; f < 5; f++) { xp_prev[f] = 0.; Fx_prev[f] = 0.f; ly[f] = 0.f; lz[f] = 0.f; }
+    for (int f = 0; f < 4; f++) xp_prev.f[f] = 0.f;
+    xp_prev.E = 0.;
+#pragma unroll
+    for (int f = 0; f < 5; f++) { Fx_prev[f] = 0.f; ly[f] = 0.f; lz[f] = 0.f; }
     {
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
         load5(a.in, (long long)ia * si + col, w1);
@@ -125,41 +130,44 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         }
         mx::PrimF P;
         mx::cell_prims(w1, P);
+        const float qf[4] = {(float)w1[0], (float)w1[1], (float)w1[2], (float)w1[3]};
         const bool inseg = (i >= i0 && i < i1);
-        double yp[5], zp[5];
+        mx::FaceF yp, zp;
         if (inseg) {
             const long long pl = (long long)i * si;
-            double qp[5], qm[5], mi5[5];
+            double qp[5], qm[5];
+            mx::FaceF mi;
             load5(a.in, pl + (long long)jp * sj + kc, qp);
             load5(a.in, pl + (long long)jm * sj + kc, qm);
-            mx::recon_dir<2, 3, 1>(P, w1, qp, qm, false, yp, mi5);
+            mx::recon_dir<2, 3, 1>(P, w1, qf, qp, qm, false, yp, mi);
 #pragma unroll
-            for (int f = 0; f < 5; f++) s_ym[f][c] = mi5[f];
+            for (int f = 0; f < 4; f++) s_ym[f][c] = mi.f[f];
+            s_ymE[c] = mi.E;
             load5(a.in, pl + (long long)jc * sj + zkp, qp);
             load5(a.in, pl + (long long)jc * sj + zkm, qm);
-            mx::recon_dir<3, 1, 2>(P, w1, qp, qm, a.ph.z_quirk != 0, zp, mi5);
+            mx::recon_dir<3, 1, 2>(P, w1, qf, qp, qm, a.ph.z_quirk != 0, zp, mi);
 #pragma unroll
-            for (int f = 0; f < 5; f++) s_zm[f][c] = mi5[f];
+            for (int f = 0; f < 4; f++) s_zm[f][c] = mi.f[f];
+            s_zmE[c] = mi.E;
         }
-        double xp[5], xm[5];
-        mx::recon_dir<1, 2, 3>(P, w1, w2, w0, false, xp, xm);
+        mx::FaceF xp, xm;
+        mx::recon_dir<1, 2, 3>(P, w1, qf, w2, w0, false, xp, xm);
         __syncthreads();
 
         if (i >= i0 && interior) {
-            double l[5], r[5];
+            mx::FaceF l, r;
             if (i == 1) {
-                inflow_face(a.b, H, j - 1, k - 1, l);
+                double q[5];
+                inflow_face(a.b, H, j - 1, k - 1, q);
+                l = mx::to_face(q);
             } else {
-#pragma unroll
-                for (int f = 0; f < 5; f++) l[f] = xp_prev[f];
+                l = xp_prev;
             }
             if (i == L + 1) {
-#pragma unroll
-                for (int f = 0; f < 5; f++) r[f] = xp_prev[f];
-                outflow_state(a.b.p_out, r);
+                r = xp_prev;
+                mx::outflow_face(a.b.p_out, r);
             } else {
-#pragma unroll
-                for (int f = 0; f < 5; f++) r[f] = xm[f];
+                r = xm;
             }
             float R, vel[3], Pf;
             mx::riemann<0>(l, r, R, vel, Pf);
@@ -196,30 +204,29 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
             for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
         }
-#pragma unroll
-        for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
+        xp_prev = xp;
 
         if (inseg) {
             const int rp = ring(i + 1), rc = ring(i), rm = ring(i - 1);
             const float vbk[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
             const float Tb = s_visc[vc][3][c], Rb = s_visc[vc][4][c];
             if (yface) {
-                double l[5], r[5];
+                mx::FaceF up;   // - state of the face above: the neighbour's
+#pragma unroll
+                for (int f = 0; f < 4; f++) up.f[f] = s_ym[f][c + BK];
+                up.E = s_ymE[c + BK];
+                mx::FaceF l, r;
                 if (j == 0) {
-#pragma unroll
-                    for (int f = 0; f < 5; f++) l[f] = s_ym[f][c + BK];
-                    l[2] = -l[2];
+                    l = up;
+                    l.f[2] = -l.f[2];
                 } else {
-#pragma unroll
-                    for (int f = 0; f < 5; f++) l[f] = yp[f];
+                    l = yp;
                 }
                 if (j == H) {
-#pragma unroll
-                    for (int f = 0; f < 5; f++) r[f] = yp[f];
-                    r[2] = -r[2];
+                    r = yp;
+                    r.f[2] = -r.f[2];
                 } else {
-#pragma unroll
-                    for (int f = 0; f < 5; f++) r[f] = s_ym[f][c + BK];
+                    r = up;
                 }
                 float R, vel[3], Pf;
                 mx::riemann<1>(l, r, R, vel, Pf);
@@ -242,9 +249,11 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 for (int f = 0; f < 5; f++) s_fy[f][c] = Fy[f];
             }
             if (zface) {
-                double l[5], r[5];
+                const mx::FaceF l = zp;
+                mx::FaceF r;
 #pragma unroll
-                for (int f = 0; f < 5; f++) { l[f] = zp[f]; r[f] = s_zm[f][c + 1]; }
+                for (int f = 0; f < 4; f++) r.f[f] = s_zm[f][c + 1];
+                r.E = s_zmE[c + 1];
                 float R, vel[3], Pf;
                 mx::riemann<2>(l, r, R, vel, Pf);
                 const float vf[3] = {s_visc[vc][0][c + 1], s_visc[vc][1][c + 1], s_visc[vc][2][c + 1]};
@@ -283,13 +292,8 @@ void launch_mixed(hipStream_t s, int stage, MixedArgs a, int per_cu)
     a.ntj = (a.g.H + TJ - 1) / TJ;
     a.ntk = (a.g.D + TK - 1) / TK;
     const int cols = a.ntj * a.ntk;
-    const int target = 4 * 256 * per_cu;
-    int nseg = (target + cols - 1) / cols;
-    if (nseg > a.g.L) nseg = a.g.L;
-    if (nseg < 1) nseg = 1;
-    int segL = (a.g.L + nseg - 1) / nseg;
-    if (segL < 4 && a.g.L >= 4) segL = 4;
-    nseg = (a.g.L + segL - 1) / segL;
+    int segL = 0;
+    const int nseg = fused_segments(a.g.L, cols, per_cu, &segL);
     a.segL = segL;
     a.nseg = nseg;
     a.nblocks = nseg * cols;

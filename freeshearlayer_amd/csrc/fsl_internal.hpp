@@ -80,6 +80,9 @@ void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Ge
 void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
                     void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
 
+// x segments of the fused stage kernels (fsl_fused.hip): count and length
+int fused_segments(int L, int cols, int per_cu, int* seg_len);
+
 // --- learned closure (fsl_mlp.hip) ---
 int mlp_nparam();
 int mlp_npack();   // packed BF16 operand entries (8 bytes each)
