@@ -336,4 +336,40 @@ __device__ __forceinline__ void outflow_state(double p_out, double q[5])
     q[4] = p_out / kK_1 + 0.5 * (u * u + v * v + w * w) / R;
 }
 
+// Per-cell quantity of the CFL reduction: mode 0 the reference's signed
+// streamwise Courant number (rho u / rho) dt / dx (src/helpers.c:23-52),
+// mode 1 the acoustic rate sum_d (|u_d| + c) / delta_d.
+__device__ __forceinline__ double cfl_cell(int mode, const double q[5], const Phys& ph, double dtX)
+{
+    const double R = q[0];
+    if (mode == 0) {
+        const double u = q[1] / R;
+        return u * dtX;
+    }
+    const double u = q[1] / R, vv = q[2] / R, w = q[3] / R;
+    const double p = (q[4] - 0.5 * R * (u * u + vv * vv + w * w)) * kK_1;
+    const double cs = sqrt(kK * p / R);
+    return (fabs(u) + cs) * ph.rdel[0] + (fabs(vv) + cs) * ph.rdel[1] + (fabs(w) + cs) * ph.rdel[2];
+}
+
+// The reference seeds the maximum with 0 and skips NaN (comparison false):
+// every lane contributes max(+0, value), and the maximum is folded into a
+// 64-bit atomic max on the bit pattern, which orders non-negative doubles.
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = (w > v) ? w : v;
+    }
+    return v;
+}
+
+// wave maximum of m into *out (every lane of the wave calls this)
+__device__ __forceinline__ void cfl_fold(double m, unsigned long long* out)
+{
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 }  // namespace fsl

@@ -31,6 +31,8 @@ void ctx_prof(fsl_ctx* c, int id, int end);
 int ctx_check(fsl_ctx* c, const char* what);
 int ctx_sgs(fsl_ctx* c);
 int learned_sgs_step(fsl_ctx* c);
+int ctx_cfl_reset(fsl_ctx* c);
+void ctx_cfl_ready(fsl_ctx* c);
 int ctx_ghost(fsl_ctx* c, const double* state);
 void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
@@ -47,6 +49,8 @@ struct FusedArgs {
     Bnd b;
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+    unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (or null)
+    int cfl_mode;
 };
 
 __device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
@@ -106,6 +110,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
     const long long sj = g.sj, si = g.si;
     const long long col = (long long)jc * sj + kc;   // (j, k) offset of this column
 
+    double cfl_m = 0.0;           // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];   // stage state at planes i-1, i, i+1 (own column)
     double xp_prev[5];            // + x state of plane i-1
     double Fx_prev[5];
@@ -210,6 +215,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                          s_visc[vb][4][c], gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
                 const long long cc = (long long)(i - 1) * si + col;
+                double ov[5];
 #pragma unroll
                 for (int f = 0; f < 5; f++) {
                     const double lx = a.dtX * (Fx_prev[f] - Fx[f]);
@@ -219,6 +225,11 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                     else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
                     else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
                     a.out.f[f][cc] = o;
+                    ov[f] = o;
+                }
+                if (STAGE == 3 && a.cfl_out) {
+                    const double v = cfl_cell(a.cfl_mode, ov, a.ph, a.dtX);
+                    cfl_m = (v > cfl_m) ? v : cfl_m;
                 }
             }
 #pragma unroll
@@ -310,6 +321,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
             }
         }
     }
+    if (STAGE == 3 && a.cfl_out) cfl_fold(cfl_m, a.cfl_out);
 }
 
 // Segments along x.  Workgroups of one launch run in rounds of `resident`
@@ -437,6 +449,11 @@ int fused_stage(fsl_ctx* ctx, int stage)
     a.dtX = v.dtX;
     a.dtY = v.dtY;
     a.dtZ = v.dtZ;
+    // stage 3 also reduces the CFL quantity of the new state for the next
+    // step's time step (fsl_check_co_num uses it while U and dt are unchanged)
+    a.cfl_out = stage == 3 ? v.red : nullptr;
+    a.cfl_mode = v.cfl_mode;
+    if (stage == 3 && (rc = ctx_cfl_reset(ctx))) return rc;
     ctx_prof(ctx, K_FUSED_XYZ, 0);
     if (v.precision == FSL_PREC_MIXED) {
         launch_fused_mixed(v, stage, in, out);
@@ -449,6 +466,7 @@ int fused_stage(fsl_ctx* ctx, int stage)
     ctx_prof(ctx, K_FUSED_XYZ, 1);
     rc = ctx_check(ctx, "fused stage");
     if (rc) return rc;
+    if (stage == 3) ctx_cfl_ready(ctx);
     // stage inputs: U, then Up, then Uq; the state after stage 3 is U
     ctx_set_cur(ctx, stage == 3 ? 0 : 1);
     return FSL_OK;

@@ -27,6 +27,8 @@ struct MixedArgs {
     Bnd b;
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+    unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (FP64; or null)
+    int cfl_mode;
 };
 
 __device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
@@ -82,6 +84,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
 
+    double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
     mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
@@ -190,6 +193,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                              s_visc[vb][4][c], gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
                 const long long cc = (long long)(i - 1) * si + col;
+                double ov[5];
 #pragma unroll
                 for (int f = 0; f < 5; f++) {
                     const double lx = a.dtX * ((double)Fx_prev[f] - (double)Fx[f]);
@@ -199,6 +203,11 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
                     else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
                     a.out.f[f][cc] = o;
+                    ov[f] = o;
+                }
+                if (STAGE == 3 && a.cfl_out) {
+                    const double v = cfl_cell(a.cfl_mode, ov, a.ph, a.dtX);
+                    cfl_m = (v > cfl_m) ? v : cfl_m;
                 }
             }
 #pragma unroll
@@ -283,6 +292,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             }
         }
     }
+    if (STAGE == 3 && a.cfl_out) cfl_fold(cfl_m, a.cfl_out);
 }
 
 template <int BJ, int BK, int MINW>
@@ -321,6 +331,8 @@ void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* o
     a.dtX = v.dtX;
     a.dtY = v.dtY;
     a.dtZ = v.dtZ;
+    a.cfl_out = stage == 3 ? v.red : nullptr;
+    a.cfl_mode = v.cfl_mode;
     static int minw = -1;
     if (minw < 0) {
         const char* e = getenv("FSL_MIXED_MINW");

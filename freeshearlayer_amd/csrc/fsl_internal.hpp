@@ -123,6 +123,8 @@ struct CtxView {
     int cur_is_p;
     int sgs_model;
     int precision;
+    unsigned long long* red;   // CFL reduction cell
+    int cfl_mode;
 };
 void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* out);
 

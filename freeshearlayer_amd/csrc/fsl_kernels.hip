@@ -277,20 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(State5 U, State5 Up, Faces F, Geo
 }
 
 // ---------------------------------------------------------------------
-// CFL reductions.  The reference seeds the maximum with 0 and skips NaN
-// (comparison false), so every lane contributes max(+0, value) and the
-// block maximum is folded into a 64-bit atomic max on the bit pattern,
-// which orders non-negative doubles correctly.
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = (w > v) ? w : v;
-    }
-    return v;
-}
-
+// CFL reduction (cfl_cell, wave_max: fsl_device.hpp)
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cfl(CState5 U, Geo g, Phys ph, double dtX,
                                                 unsigned long long* out)
