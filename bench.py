@@ -77,9 +77,24 @@ def algorithmic_bytes(kernel, L, H, D, sgs):
     return per.get(kernel)
 
 
-# MFMA work of the learned closure per cell: 64-cell tiles through
-# 16x64 (K padded 8 -> 16), 64x64 and 64x16 (w3 in column 0) products
-MLP_FLOP_PER_CELL = 2 * (16 * 64 + 64 * 64 + 64 * 16)
+# Learned closure per cell: 4 672 multiply-adds of the network (8x64 + 64x64
+# + 64x1) = 9 344 algorithmic FLOPs; the MFMA tiles execute 12 288 (K of the
+# first layer padded 8 -> 16, w3 in column 0 of a 16-wide tile)
+MLP_FLOP_PER_CELL = 2 * (8 * 64 + 64 * 64 + 64 * 1)
+MLP_MFMA_FLOP_PER_CELL = 2 * (16 * 64 + 64 * 64 + 64 * 16)
+
+
+def host_cpu():
+    """CPU model and logical core count of the host."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def workload_name(precision, sgs, path, L, H, D, mach):
@@ -159,10 +174,11 @@ def cpu_baseline(steps=2, mesh=(192, 128, 96)):
         orc.step()
     t = time.perf_counter() - t0
     orc.close()
+    model, ncpu = host_cpu()
     return {"value": round(L * H * D * 3 * steps / t / 1e6, 4), "unit": UNIT, "cores": 1, "kind": "port",
             "sample": "%d steps of the FP64 dynamic-SGS CPU oracle on the same shear layer at %dx%dx%d "
                       "(%.1f s)" % (steps, L, H, D, t),
-            "cpu": platform.processor() or platform.machine()}
+            "cpu": model, "host_logical_cores": ncpu}
 
 
 def main(argv=None):
@@ -214,7 +230,8 @@ def main(argv=None):
         tab = profile_table()
         mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
-                        "flop_per_cell": MLP_FLOP_PER_CELL, "avg_launch_us": round(ms / cnt * 1e3, 2),
+                        "flop_per_cell": MLP_FLOP_PER_CELL, "mfma_flop_per_cell": MLP_MFMA_FLOP_PER_CELL,
+                        "avg_launch_us": round(ms / cnt * 1e3, 2),
                         "mfma_busy_pct": tab.get("mfma_busy@mlp_mu@%s" % wl),
                         "valu_busy_pct": tab.get("valu_busy@mlp_mu@%s" % wl)}
     final_dt = ctx.dt

@@ -233,7 +233,6 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         delete c;
         return FSL_ERR_ARG;
     }
-    c->sgs_valid_step = -1;
     const int L = cs->LEN, H = cs->HIG, D = cs->DEP;
     c->g.L = L; c->g.H = H; c->g.D = D;
     c->g.sj = D + 2;
@@ -371,7 +370,6 @@ int fsl_upload_state(fsl_ctx* c, int which, const double* U)
     double* d = which ? c->Up : c->U;
     HIPCHK(c, hipMemcpyAsync(d, U, 5 * c->g.G * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->sgs_valid_step = -1;
     return FSL_OK;
 }
 

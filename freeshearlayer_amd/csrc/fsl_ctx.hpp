@@ -75,7 +75,6 @@ struct fsl_ctx {
     double dt, dtX, dtY, dtZ;
     fsl_block blk;
     unsigned step;
-    int sgs_valid_step; // step whose mu_SGS is current (fused path), -1 none
     int cfl_ready;      // c->red holds the CFL maximum of U (from fused stage 3) for dtX
 
     int prof;

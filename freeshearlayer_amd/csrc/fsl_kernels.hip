@@ -60,12 +60,19 @@ __global__ void __launch_bounds__(kBlock) k_ghost_fill(State5 U, Geo g, Bnd b)
         j = side ? H + 1 : 0;
         i = 1 + (int)(r / (D + 2));
         k = (int)(r % (D + 2));
-    } else if ((t -= 2 * py) < 2 * pz) {
-        const int side = (int)(t / pz);
-        const long long r = t - side * pz;
-        k = side ? D + 1 : 0;
-        i = 1 + (int)(r / H);
-        j = 1 + (int)(r % H);
+    } else if ((t -= 2 * py) < pz) {
+        // z periodic ghosts of one interior row (i, j): k = 0 <- D and k = D+1 <- 1.
+        // One thread per row touches the row's first and last cache lines once.
+        i = 1 + (int)(t / H);
+        j = 1 + (int)(t % H);
+        const long long row = cidx(g, i, j, 0);
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const double lo = U.f[f][row + D], hi = U.f[f][row + 1];
+            U.f[f][row] = lo;
+            U.f[f][row + D + 1] = hi;
+        }
+        return;
     } else {
         return;
     }
@@ -368,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) k_copy(const double* __restrict__ a, d
 // ---------------------------------------------------------------------
 void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b)
 {
-    const long long n = 2LL * (g.H + 2) * (g.D + 2) + 2LL * g.L * (g.D + 2) + 2LL * g.L * g.H;
+    const long long n = 2LL * (g.H + 2) * (g.D + 2) + 2LL * g.L * (g.D + 2) + (long long)g.L * g.H;
     hipLaunchKernelGGL(k_ghost_fill, dim3(nblk(n)), dim3(kBlock), 0, s, U, g, b);
 }
 

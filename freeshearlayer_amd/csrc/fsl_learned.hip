@@ -241,7 +241,6 @@ int learned_sgs_step(fsl_ctx* c)
         if ((rc = learned_mu(c, c->mu))) return rc;
         L.learned_steps++;
     }
-    c->sgs_valid_step = (int)c->step;
     return FSL_OK;
 }
 
@@ -274,7 +273,6 @@ int fsl_mlp_set_params(fsl_ctx* c, const float* params, const float* norm)
         HIPCHK(c, hipMemsetAsync(L.grad, 0, nb, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    c->sgs_valid_step = -1;
     return FSL_OK;
 }
 
@@ -378,7 +376,6 @@ int fsl_learned_smagorinsky(fsl_ctx* c)
     int rc = need_ready(c);
     if (rc) return rc;
     rc = learned_mu(c, c->mu);
-    c->sgs_valid_step = -1;
     return rc;
 }
 
@@ -429,7 +426,6 @@ int fsl_mlp_collect(fsl_ctx* c, long long stride, long long offset, long long* n
     if ((rc = ctx_ghost(c, c->U))) return rc;
     if ((rc = update_thickness(c))) return rc;
     if ((rc = dynamic_mu(c, L.cd))) return rc;
-    c->sgs_valid_step = -1;
     if (nn > 0) {
         launch_mlp_features(c->stream, cst5(c->U, c->g.G), c->g, c->ph, c->dU, L.dw, c->cs.deltaY, L.cd, stride,
                             offset, L.X + L.n * 8, L.t + L.n, nn);
@@ -552,7 +548,6 @@ int fsl_mlp_train(fsl_ctx* c, int epochs, long long batch, float lr, unsigned lo
         if (epoch_loss) epoch_loss[e] = loss / (float)(steps * batch);
         if (!isfinite(loss)) return set_err(c, FSL_ERR_NONFINITE, "training diverged in epoch %d", e);
     }
-    c->sgs_valid_step = -1;
     return FSL_OK;
 }
 
@@ -627,7 +622,6 @@ int fsl_learned_judge(fsl_ctx* c, double* err, double* d_dynamic, double* d_lear
     if (rc) return rc;
     if ((rc = ensure_judge(c)) || (rc = update_thickness(c))) return rc;
     if ((rc = dynamic_mu(c, nullptr)) || (rc = learned_mu(c, c->lrn.mu_alt))) return rc;
-    c->sgs_valid_step = -1;
     double e, dd, dl;
     if ((rc = judge_error(c, c->mu, c->lrn.mu_alt, &e, &dd, &dl))) return rc;
     if (err) *err = e;
