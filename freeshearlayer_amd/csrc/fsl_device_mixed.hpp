@@ -25,6 +25,12 @@ namespace mx {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Limiter of src/reconstruction.c:17-24 as the median of {x, 0, 4y}: for x and
+// y of one sign it is the one of x and 4y nearer zero, for opposite signs or a
+// zero operand it is 0 -- the reference's cases, in one v_med3_f32 (no
+// compare/select chain).
+__device__ __forceinline__ float minmod(float x, float y) { return __builtin_amdgcn_fmed3f(x, 0.f, 4.f * y); }
+
 // pressure from FP64 total energy and an FP32 kinetic energy
 __device__ __forceinline__ float pressure(double E, float ke) { return (float)((E - (double)ke) * kK_1); }
 
