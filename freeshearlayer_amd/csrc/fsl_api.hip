@@ -238,6 +238,12 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
     c->g.sj = D + 2;
     c->g.si = (long long)(H + 2) * (D + 2);
     c->g.G = (long long)(L + 2) * c->g.si;
+    if (c->cfg.precision == FSL_PREC_MIXED && c->g.si * 8 > 0x7fffffffLL) {
+        // the mixed stage kernel addresses one (j, k) plane of a field with 32-bit byte offsets
+        fprintf(stderr, "fsl_create: a %d x %d plane is too large for the mixed-precision path\n", H, D);
+        delete c;
+        return FSL_ERR_UNSUPPORTED;
+    }
     c->nface[0] = (size_t)(L + 1) * H * D;
     c->nface[1] = (size_t)L * (H + 1) * D;
     c->nface[2] = (size_t)L * H * (D + 1);

@@ -3,9 +3,10 @@
 // The decomposition of fsl_fused.hip (16 x 16 workgroup tile with a one-cell
 // halo ring marching along x; x faces in registers, y/z faces and viscous
 // data through LDS) with the FP32 arithmetic of fsl_device_mixed.hpp.  The
-// state, the face states, pressure recovery and the RK accumulation stay in
-// FP64; fluxes, 1/rho-velocities and viscous cell data travel through LDS in
-// FP32, which halves the kernel's register and LDS footprint.
+// state, face energies, pressure recovery and the RK accumulation U + dU stay
+// in FP64; the increment dU = -dt div F is summed in FP32 from the FP32
+// fluxes.  Fluxes, 1/rho-velocities and viscous cell data travel through LDS
+// in FP32, which halves the kernel's register and LDS footprint.
 #include <stdlib.h>
 
 #include "fsl.h"
@@ -31,10 +32,33 @@ struct MixedArgs {
     int cfl_mode;
 };
 
-__device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
+// Plane-relative buffer access.  The x index of a plane is the same for the
+// whole workgroup, so each field's plane is one buffer resource in scalar
+// registers (base = field + plane, range = one plane) and a lane supplies only
+// its 32-bit in-plane byte offset: no per-lane 64-bit address arithmetic, and
+// the hardware range check confines every access to that plane.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const double* field, long long plane, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(field + plane), (short)0, bytes, kRsrcWord3);
+}
+
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+}
+
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned boff, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)boff, 0, 0);
+}
+
+__device__ __forceinline__ void load5(const CState5& s, long long plane, int pbytes, unsigned boff, double q[5])
 {
 #pragma unroll
-    for (int f = 0; f < 5; f++) q[f] = s.f[f][c];
+    for (int f = 0; f < 5; f++) q[f] = bload(plane_rsrc(s.f[f], plane, pbytes), boff);
 }
 
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
@@ -79,10 +103,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     int zkm = kc - 1, zkp = kc + 1;
     if (kc == 0) { zkm = D - 1; zkp = D + 1; }
     else if (kc == D + 1) { zkm = 0; zkp = 2; }
-    const long long sj = g.sj, si = g.si;
-    const long long col = (long long)jc * sj + kc;
+    const long long si = g.si;
+    const int pb = (int)(si * 8);                         // bytes of one plane of one field
+    const unsigned sjb = (unsigned)g.sj * 8u;            // byte stride of j within a plane
+    const unsigned colb = (unsigned)jc * sjb + (unsigned)kc * 8u;
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
+    const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
@@ -95,8 +122,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     for (int f = 0; f < 5; f++) { Fx_prev[f] = 0.f; ly[f] = 0.f; lz[f] = 0.f; }
     {
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
-        load5(a.in, (long long)ia * si + col, w1);
-        load5(a.in, (long long)ib * si + col, w2);
+        load5(a.in, (long long)ia * si, pb, colb, w1);
+        load5(a.in, (long long)ib * si, pb, colb, w2);
         const float ri = mx::rcp((float)w2[0]);
         const int r = ring(ib);
         s_vr[r][0][c] = (float)w2[1] * ri;
@@ -107,13 +134,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     for (int i = i0 - 1; i <= i1; i++) {
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
-        load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
+        load5(a.in, (long long)min(i + 1, L + 1) * si, pb, colb, w2);
         // step state of the cell this iteration updates (i - 1), issued early:
         // its use after the last barrier would otherwise wait for the load
         double bq[5];
-        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si + col, bq);
+        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
-        const double mu_c = a.ph.dynamic ? a.mu[(long long)i * si + col] : 0.;
+        const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colb) : 0.;
         __syncthreads();
         const int vc = i & 1, vb = vc ^ 1;
         {
@@ -140,14 +167,14 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             const long long pl = (long long)i * si;
             double qp[5], qm[5];
             mx::FaceF mi;
-            load5(a.in, pl + (long long)jp * sj + kc, qp);
-            load5(a.in, pl + (long long)jm * sj + kc, qm);
+            load5(a.in, pl, pb, (unsigned)jp * sjb + (unsigned)kc * 8u, qp);
+            load5(a.in, pl, pb, (unsigned)jm * sjb + (unsigned)kc * 8u, qm);
             mx::recon_dir<2, 3, 1>(P, w1, qf, qp, qm, false, yp, mi);
 #pragma unroll
             for (int f = 0; f < 4; f++) s_ym[f][c] = mi.f[f];
             s_ymE[c] = mi.E;
-            load5(a.in, pl + (long long)jc * sj + zkp, qp);
-            load5(a.in, pl + (long long)jc * sj + zkm, qm);
+            load5(a.in, pl, pb, (unsigned)jc * sjb + (unsigned)zkp * 8u, qp);
+            load5(a.in, pl, pb, (unsigned)jc * sjb + (unsigned)zkm * 8u, qm);
             mx::recon_dir<3, 1, 2>(P, w1, qf, qp, qm, a.ph.z_quirk != 0, zp, mi);
 #pragma unroll
             for (int f = 0; f < 4; f++) s_zm[f][c] = mi.f[f];
@@ -192,17 +219,19 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             mx::face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
                              s_visc[vb][4][c], gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
-                const long long cc = (long long)(i - 1) * si + col;
+                const long long po = (long long)(i - 1) * si;
                 double ov[5];
 #pragma unroll
                 for (int f = 0; f < 5; f++) {
-                    const double lx = a.dtX * ((double)Fx_prev[f] - (double)Fx[f]);
-                    const double s = w0[f] + lx + (double)ly[f] + (double)lz[f];
+                    // the increment -dt div F in FP32 (its own rounding is below that of
+                    // the FP32 fluxes), accumulated onto the FP64 state
+                    const float inc = dtXf * (Fx_prev[f] - Fx[f]) + ly[f] + lz[f];
+                    const double s = w0[f] + (double)inc;
                     double o;
                     if (STAGE == 1) o = s;
                     else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
                     else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
-                    a.out.f[f][cc] = o;
+                    bstore(plane_rsrc(a.out.f[f], po, pb), colb, o);
                     ov[f] = o;
                 }
                 if (STAGE == 3 && a.cfl_out) {
@@ -287,8 +316,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         if (inseg && interior) {
 #pragma unroll
             for (int f = 0; f < 5; f++) {
-                ly[f] = (float)(a.dtY * ((double)s_fy[f][c - BK] - (double)s_fy[f][c]));
-                lz[f] = (float)(a.dtZ * ((double)s_fz[f][c - 1] - (double)s_fz[f][c]));
+                ly[f] = dtYf * (s_fy[f][c - BK] - s_fy[f][c]);
+                lz[f] = dtZf * (s_fz[f][c - 1] - s_fz[f][c]);
             }
         }
     }
