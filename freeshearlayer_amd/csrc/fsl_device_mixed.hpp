@@ -154,38 +154,54 @@ __device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], con
     minus.E = q[4] - (double)d[4];
 }
 
+// One side of a face for the Riemann solver: velocity, pressure (from the
+// FP64 energy), sound speed C, t = 1/C, A = 1/(rho C) and p/(rho C).
+struct SideF {
+    float v[3], P, C, t, A, PA;
+};
+
+__device__ __forceinline__ SideF face_side(const FaceF& q)
+{
+#pragma clang fp contract(fast)
+    SideF s;
+    const float R = q.f[0], Ri = rcp(R);
+    s.v[0] = q.f[1] * Ri; s.v[1] = q.f[2] * Ri; s.v[2] = q.f[3] * Ri;
+    s.P = pressure(q.E, 0.5f * R * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]));
+    const float C2 = 1.4f * s.P * Ri;
+    s.t = __builtin_amdgcn_rsqf(C2);
+    s.C = C2 * s.t;
+    s.A = Ri * s.t;
+    s.PA = s.P * s.A;
+    return s;
+}
+
+// Linearised Riemann solver of src/fluxes.c:129-170 (as fsl_device.hpp's
+// riemann), written with C^2 = K p / rho so that its terms need one
+// reciprocal and one reciprocal square root per side:
+//   RU = p / (rho C)                    (the Riemann invariants J+- = u_n +- RU)
+//   al+- = (K-1)/2 (J-+ - J+-) / J_o = +-RU / p = +-1 / (rho C),  J_o = -(K-1) p
+//   al_o = -(K/2 (J+ - J-))^2 = -C^2, so (J_o - P) / al_o = (P - J_o) t^2
+// Same upwind choices (on u_n + C_m, u_n - C_m, u_n and the new u_n).
 template <int N>
 __device__ __forceinline__ void riemann(const FaceF& l, const FaceF& r, float& Ro, float vel[3], float& Po)
 {
 #pragma clang fp contract(fast)
-    float lv[3], rv[3];
-    const float lR = l.f[0], rR = r.f[0];
-    const float lRi = rcp(lR), rRi = rcp(rR);
-    lv[0] = l.f[1] * lRi; lv[1] = l.f[2] * lRi; lv[2] = l.f[3] * lRi;
-    rv[0] = r.f[1] * rRi; rv[1] = r.f[2] * rRi; rv[2] = r.f[3] * rRi;
-    const float lP = pressure(l.E, 0.5f * lR * (lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]));
-    const float rP = pressure(r.E, 0.5f * rR * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]));
-    const float lC = fsqrt(1.4f * lP * lRi), rC = fsqrt(1.4f * rP * rRi);
-    const float ljo = -0.4f * lP, rjo = -0.4f * rP;
-    const float lRU = lP * rcp(lR * lC), rRU = rP * rcp(rR * rC);
-    const float ljp = lv[N] + lRU, ljm = lv[N] - lRU;
-    const float rjp = rv[N] + rRU, rjm = rv[N] - rRU;
-    float Un = 0.5f * (lv[N] + rv[N]);
-    const float Cm = 0.5f * (lC + rC);
-    const float C_p = Un + Cm, C_m = Un - Cm, C_o = Un;
-    float al_p, al_m, al_o, J_p, J_m, J_o, a;
-    if (C_p > 0.f) { al_p = 0.2f * (ljm - ljp) * rcp(ljo); J_p = ljp; }
-    else           { al_p = 0.2f * (rjm - rjp) * rcp(rjo); J_p = rjp; }
-    if (C_m > 0.f) { al_m = 0.2f * (ljp - ljm) * rcp(ljo); J_m = ljm; }
-    else           { al_m = 0.2f * (rjp - rjm) * rcp(rjo); J_m = rjm; }
-    if (C_o > 0.f) { a = 0.7f * (ljp - ljm); al_o = -a * a; J_o = ljo; }
-    else           { a = 0.7f * (rjp - rjm); al_o = -a * a; J_o = rjo; }
-    const float P = (J_p - J_m) * rcp(al_p - al_m);
+    const SideF L = face_side(l), Rs = face_side(r);
+    float Un = 0.5f * (L.v[N] + Rs.v[N]);
+    const float Cm = 0.5f * (L.C + Rs.C);
+    const bool sp = Un + Cm > 0.f, sm = Un - Cm > 0.f, so = Un > 0.f;
+    const float al_p = sp ? L.A : Rs.A;
+    const float J_p = sp ? L.v[N] + L.PA : Rs.v[N] + Rs.PA;
+    const float am = sm ? L.A : Rs.A;                     // al_m = -am
+    const float J_m = sm ? L.v[N] - L.PA : Rs.v[N] - Rs.PA;
+    const float to = so ? L.t : Rs.t;
+    const float J_o = -0.4f * (so ? L.P : Rs.P);
+    const float P = (J_p - J_m) * rcp(al_p + am);
     Un = J_p - al_p * P;
-    Ro = (J_o - P) * rcp(al_o);
+    Ro = (P - J_o) * to * to;
     const bool left = Un > 0.f;
 #pragma unroll
-    for (int t = 0; t < 3; t++) vel[t] = left ? lv[t] : rv[t];
+    for (int t = 0; t < 3; t++) vel[t] = left ? L.v[t] : Rs.v[t];
     vel[N] = Un;
     Po = P;
 }

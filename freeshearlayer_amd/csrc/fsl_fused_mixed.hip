@@ -107,12 +107,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const int pb = (int)(si * 8);                         // bytes of one plane of one field
     const unsigned sjb = (unsigned)g.sj * 8u;            // byte stride of j within a plane
     const unsigned colb = (unsigned)jc * sjb + (unsigned)kc * 8u;
+    const unsigned oyp = (unsigned)jp * sjb + (unsigned)kc * 8u, oym = (unsigned)jm * sjb + (unsigned)kc * 8u;
+    const unsigned ozp = (unsigned)jc * sjb + (unsigned)zkp * 8u, ozm = (unsigned)jc * sjb + (unsigned)zkm * 8u;
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
+    double pyp[5], pym[5], pzp[5], pzm[5];   // y/z neighbours of the cell of the next plane
     mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
 #pragma unroll
@@ -124,6 +127,10 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
         load5(a.in, (long long)ia * si, pb, colb, w1);
         load5(a.in, (long long)ib * si, pb, colb, w2);
+        load5(a.in, (long long)ib * si, pb, oyp, pyp);
+        load5(a.in, (long long)ib * si, pb, oym, pym);
+        load5(a.in, (long long)ib * si, pb, ozp, pzp);
+        load5(a.in, (long long)ib * si, pb, ozm, pzm);
         const float ri = mx::rcp((float)w2[0]);
         const int r = ring(ib);
         s_vr[r][0][c] = (float)w2[1] * ri;
@@ -135,13 +142,29 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
         load5(a.in, (long long)min(i + 1, L + 1) * si, pb, colb, w2);
+        // y and z neighbours of this plane's cell (loaded one iteration ahead)
+        double nyp[5], nym[5], nzp[5], nzm[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) { nyp[f] = pyp[f]; nym[f] = pym[f]; nzp[f] = pzp[f]; nzm[f] = pzm[f]; }
+        {
+            const long long pn = (long long)min(i + 1, L + 1) * si;
+            load5(a.in, pn, pb, oyp, pyp);
+            load5(a.in, pn, pb, oym, pym);
+            load5(a.in, pn, pb, ozp, pzp);
+            load5(a.in, pn, pb, ozm, pzm);
+        }
         // step state of the cell this iteration updates (i - 1), issued early:
         // its use after the last barrier would otherwise wait for the load
         double bq[5];
         if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
         const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colb) : 0.;
-        __syncthreads();
+        // No barrier here: the LDS written before the next barrier (s_vr of
+        // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
+        // minus states) was last read by the previous plane's face loops,
+        // which every thread finished before the previous plane's last
+        // barrier; s_fy/s_fz, read after that barrier, are rewritten only
+        // after the next one.
         const int vc = i & 1, vb = vc ^ 1;
         {
             const float ri = mx::rcp((float)w2[0]);
@@ -164,18 +187,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const bool inseg = (i >= i0 && i < i1);
         mx::FaceF yp, zp;
         if (inseg) {
-            const long long pl = (long long)i * si;
-            double qp[5], qm[5];
             mx::FaceF mi;
-            load5(a.in, pl, pb, (unsigned)jp * sjb + (unsigned)kc * 8u, qp);
-            load5(a.in, pl, pb, (unsigned)jm * sjb + (unsigned)kc * 8u, qm);
-            mx::recon_dir<2, 3, 1>(P, w1, qf, qp, qm, false, yp, mi);
+            mx::recon_dir<2, 3, 1>(P, w1, qf, nyp, nym, false, yp, mi);
 #pragma unroll
             for (int f = 0; f < 4; f++) s_ym[f][c] = mi.f[f];
             s_ymE[c] = mi.E;
-            load5(a.in, pl, pb, (unsigned)jc * sjb + (unsigned)zkp * 8u, qp);
-            load5(a.in, pl, pb, (unsigned)jc * sjb + (unsigned)zkm * 8u, qm);
-            mx::recon_dir<3, 1, 2>(P, w1, qf, qp, qm, a.ph.z_quirk != 0, zp, mi);
+            mx::recon_dir<3, 1, 2>(P, w1, qf, nzp, nzm, a.ph.z_quirk != 0, zp, mi);
 #pragma unroll
             for (int f = 0; f < 4; f++) s_zm[f][c] = mi.f[f];
             s_zmE[c] = mi.E;
