@@ -61,24 +61,28 @@ struct PrimF {
     float vhh[3], vee[3];
 };
 
-__device__ __forceinline__ void cell_prims(const double q[5], PrimF& c)
+// Per-cell quantities of the characteristic transform from the state and its
+// 1/rho and velocity (computed once per cell by the caller); C and 1/C come
+// from one reciprocal square root of K p / rho.
+__device__ __forceinline__ void cell_prims(const double q[5], float Ri, const float v[3], PrimF& c)
 {
 #pragma clang fp contract(fast)
     const float r = (float)q[0];
-    c.R = rcp(r);
-    const float m0 = (float)q[1], m1 = (float)q[2], m2 = (float)q[3];
-    c.vel[0] = m0 * c.R;
-    c.vel[1] = m1 * c.R;
-    c.vel[2] = m2 * c.R;
+    c.R = Ri;
+    c.vel[0] = v[0];
+    c.vel[1] = v[1];
+    c.vel[2] = v[2];
     const float ff = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
     c.P = pressure(q[4], 0.5f * r * ff);
-    c.C = fsqrt(1.4f * c.P * c.R);
+    const float C2 = 1.4f * c.P * c.R;
+    const float rC = __builtin_amdgcn_rsqf(C2);
+    c.C = C2 * rC;
     c.aa = 0.2f * ff;
-    c.ll = 0.5f * rcp(c.C);
-    c.ee = c.ll * rcp(c.C);
+    c.ll = 0.5f * rC;
+    c.ee = c.ll * rC;
     c.gg = -ff * c.ee;
     c.hh = -(c.ee + c.ee);
-    c.S11 = c.aa - c.C * c.C;
+    c.S11 = c.aa - C2;
 #pragma unroll
     for (int r2 = 0; r2 < 3; r2++) {
         c.vhh[r2] = c.vel[r2] * c.hh;
@@ -87,17 +91,20 @@ __device__ __forceinline__ void cell_prims(const double q[5], PrimF& c)
     c.nn = -0.5f * c.gg;
 }
 
+// Characteristic reconstruction in direction N from the forward and backward
+// differences dp = q(+1) - q, dm = q - q(-1) of the five fields (field order,
+// FP64 differences rounded once).
 template <int N, int T1, int T2>
-__device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], const float qf[4], const double qp[5],
-                                          const double qm[5], bool quirk, FaceF& plus, FaceF& minus)
+__device__ __forceinline__ void recon_dir_d(const PrimF& c, const double q[5], const float qf[4], const float dp[5],
+                                            const float dm[5], bool quirk, FaceF& plus, FaceF& minus)
 {
 #pragma clang fp contract(fast)
     const int perm[5] = {0, N, T1, T2, 4};
     float m[5], w[5], M[5], W[5], Mh[5], Wh[5], kv[5];
 #pragma unroll
     for (int r = 0; r < 5; r++) {
-        m[r] = (float)(qp[perm[r]] - q[perm[r]]);
-        w[r] = (float)(q[perm[r]] - qm[perm[r]]);
+        m[r] = dp[perm[r]];
+        w[r] = dm[perm[r]];
     }
     const float bb = -0.4f * c.vel[N - 1];
     const float cc = -0.4f * c.vel[T1 - 1];
@@ -106,19 +113,19 @@ __device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], con
     float kk = c.C * c.vel[N - 1];
     const float S41 = c.aa - kk, S51 = c.aa + kk;
     const float S12 = bb, S42 = bb + c.C, S52 = bb - c.C;
-    float k2, k3;
-    kk = 0.4f * m[4]; k2 = cc * m[2]; k3 = dd * m[3];
-    M[0] = c.S11 * m[0] + S12 * m[1] + k2 + k3 + kk;
+    // rows 1, 4, 5 of the left eigenvectors share the terms of columns 3-5
+    float t = dd * m[3] + (cc * m[2] + 0.4f * m[4]);
+    M[0] = c.S11 * m[0] + (S12 * m[1] + t);
     M[1] = -c.vel[T1 - 1] * m[0] + m[2];
     M[2] = -c.vel[T2 - 1] * m[0] + m[3];
-    M[3] = S41 * m[0] + S42 * m[1] + k2 + k3 + kk;
-    M[4] = S51 * m[0] + S52 * m[1] + k2 + k3 + kk;
-    kk = 0.4f * w[4]; k2 = cc * w[2]; k3 = dd * w[3];
-    W[0] = c.S11 * w[0] + S12 * w[1] + k2 + k3 + kk;
+    M[3] = S41 * m[0] + (S42 * m[1] + t);
+    M[4] = S51 * m[0] + (S52 * m[1] + t);
+    t = dd * w[3] + (cc * w[2] + 0.4f * w[4]);
+    W[0] = c.S11 * w[0] + (S12 * w[1] + t);
     W[1] = -c.vel[T1 - 1] * w[0] + w[2];
     W[2] = -c.vel[T2 - 1] * w[0] + w[3];
-    W[3] = S41 * w[0] + S42 * w[1] + k2 + k3 + kk;
-    W[4] = S51 * w[0] + S52 * w[1] + k2 + k3 + kk;
+    W[3] = S41 * w[0] + (S42 * w[1] + t);
+    W[4] = S51 * w[0] + (S52 * w[1] + t);
 #pragma unroll
     for (int r = 0; r < 5; r++) {
         Mh[r] = minmod(M[r], W[r]);
@@ -152,6 +159,19 @@ __device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], con
 #pragma unroll
     for (int r = 0; r < 4; r++) minus.f[r] = qf[r] - d[r];
     minus.E = q[4] - (double)d[4];
+}
+
+template <int N, int T1, int T2>
+__device__ __forceinline__ void recon_dir(const PrimF& c, const double q[5], const float qf[4], const double qp[5],
+                                          const double qm[5], bool quirk, FaceF& plus, FaceF& minus)
+{
+    float dp[5], dm[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        dp[r] = (float)(qp[r] - q[r]);
+        dm[r] = (float)(q[r] - qm[r]);
+    }
+    recon_dir_d<N, T1, T2>(c, q, qf, dp, dm, quirk, plus, minus);
 }
 
 // One side of a face for the Riemann solver: velocity, pressure (from the
@@ -206,16 +226,8 @@ __device__ __forceinline__ void riemann(const FaceF& l, const FaceF& r, float& R
     Po = P;
 }
 
-// cell velocity, temperature (FP64 pressure recovery) and 1/rho velocity
-__device__ __forceinline__ void cell_visc(const double q[5], float v[3], float& Tc)
-{
-#pragma clang fp contract(fast)
-    const float r = (float)q[0];
-    const float ri = rcp(r);
-    v[0] = (float)q[1] * ri; v[1] = (float)q[2] * ri; v[2] = (float)q[3] * ri;
-    const float P = pressure(q[4], 0.5f * r * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-    Tc = P * rcp((float)kRair * r);
-}
+// cell temperature from the pressure and 1/rho of cell_prims
+__device__ __forceinline__ float cell_temp(const PrimF& c) { return c.P * c.R * (float)(1.0 / kRair); }
 
 template <int DIR>
 __device__ __forceinline__ void face_flux(const Phys& ph, float R, const float vel[3], float P, const float vf[3],
@@ -264,6 +276,18 @@ __device__ __forceinline__ void face_flux(const Phys& ph, float R, const float v
     F[1 + DIR] += P;
     F[4] = vel[DIR] * (3.5f * P + 0.5f * R * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) - uu * s0 -
            vv * s1 - ww * s2 + q;
+}
+
+// CFL quantity of a cell (fsl_device.hpp's cfl_cell, src/helpers.c:23-52) in
+// FP64 with one division: velocities and p / rho from 1/rho.
+__device__ __forceinline__ double cfl_cell(int mode, const double q[5], const Phys& ph, double dtX)
+{
+    const double Ri = 1.0 / q[0];
+    if (mode == 0) return q[1] * Ri * dtX;
+    const double u = q[1] * Ri, vv = q[2] * Ri, w = q[3] * Ri;
+    const double p = (q[4] - 0.5 * q[0] * (u * u + vv * vv + w * w)) * kK_1;
+    const double cs = sqrt(kK * p * Ri);
+    return (fabs(u) + cs) * ph.rdel[0] + (fabs(vv) + cs) * ph.rdel[1] + (fabs(w) + cs) * ph.rdel[2];
 }
 
 }  // namespace mx

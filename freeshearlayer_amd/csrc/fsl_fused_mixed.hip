@@ -116,6 +116,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
     double pyp[5], pym[5], pzp[5], pzm[5];   // y/z neighbours of the cell of the next plane
+    float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
+    float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
     mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
 #pragma unroll
@@ -131,11 +133,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         load5(a.in, (long long)ib * si, pb, oym, pym);
         load5(a.in, (long long)ib * si, pb, ozp, pzp);
         load5(a.in, (long long)ib * si, pb, ozm, pzm);
-        const float ri = mx::rcp((float)w2[0]);
+#pragma unroll
+        for (int f = 0; f < 5; f++) dxp[f] = (float)(w2[f] - w1[f]);
+        ri1 = mx::rcp((float)w2[0]);
         const int r = ring(ib);
-        s_vr[r][0][c] = (float)w2[1] * ri;
-        s_vr[r][1][c] = (float)w2[2] * ri;
-        s_vr[r][2][c] = (float)w2[3] * ri;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            v1[e] = (float)w2[1 + e] * ri1;
+            s_vr[r][e][c] = v1[e];
+        }
     }
 
     for (int i = i0 - 1; i <= i1; i++) {
@@ -166,23 +172,28 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // barrier; s_fy/s_fz, read after that barrier, are rewritten only
         // after the next one.
         const int vc = i & 1, vb = vc ^ 1;
+        mx::PrimF P;
+        mx::cell_prims(w1, ri1, v1, P);
         {
+            // 1/rho and velocity of plane i + 1: LDS ring, and this cell's next iteration
             const float ri = mx::rcp((float)w2[0]);
             const int r = ring(i + 1);
-            s_vr[r][0][c] = (float)w2[1] * ri;
-            s_vr[r][1][c] = (float)w2[2] * ri;
-            s_vr[r][2][c] = (float)w2[3] * ri;
-            float vd[3], Tc;
-            mx::cell_visc(w1, vd, Tc);
-            s_visc[vc][0][c] = vd[0];
-            s_visc[vc][1][c] = vd[1];
-            s_visc[vc][2][c] = vd[2];
-            s_visc[vc][3][c] = Tc;
+            float v2[3];
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                v2[e] = (float)w2[1 + e] * ri;
+                s_vr[r][e][c] = v2[e];
+            }
+            s_visc[vc][0][c] = v1[0];
+            s_visc[vc][1][c] = v1[1];
+            s_visc[vc][2][c] = v1[2];
+            s_visc[vc][3][c] = mx::cell_temp(P);
             s_visc[vc][4][c] = (float)w1[0];
             s_mu[vc][c] = mu_c;
+            ri1 = ri;
+#pragma unroll
+            for (int e = 0; e < 3; e++) v1[e] = v2[e];
         }
-        mx::PrimF P;
-        mx::cell_prims(w1, P);
         const float qf[4] = {(float)w1[0], (float)w1[1], (float)w1[2], (float)w1[3]};
         const bool inseg = (i >= i0 && i < i1);
         mx::FaceF yp, zp;
@@ -198,7 +209,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             s_zmE[c] = mi.E;
         }
         mx::FaceF xp, xm;
-        mx::recon_dir<1, 2, 3>(P, w1, qf, w2, w0, false, xp, xm);
+        {
+            // x: the backward difference is the previous plane's forward one
+            float dp[5];
+#pragma unroll
+            for (int f = 0; f < 5; f++) dp[f] = (float)(w2[f] - w1[f]);
+            mx::recon_dir_d<1, 2, 3>(P, w1, qf, dp, dxp, false, xp, xm);
+#pragma unroll
+            for (int f = 0; f < 5; f++) dxp[f] = dp[f];
+        }
         __syncthreads();
 
         if (i >= i0 && interior) {
@@ -252,7 +271,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     ov[f] = o;
                 }
                 if (STAGE == 3 && a.cfl_out) {
-                    const double v = cfl_cell(a.cfl_mode, ov, a.ph, a.dtX);
+                    const double v = mx::cfl_cell(a.cfl_mode, ov, a.ph, a.dtX);
                     cfl_m = (v > cfl_m) ? v : cfl_m;
                 }
             }
