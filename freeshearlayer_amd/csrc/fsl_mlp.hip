@@ -42,17 +42,27 @@ constexpr int PB1 = 0;                  // W1: 4 n-tiles
 constexpr int PB2 = PB1 + 4 * 64;       // W2: 4 k-steps x 4 n-tiles
 constexpr int PB2T = PB2 + 16 * 64;     // W2 transposed (backward): 4 k-steps x 4 n-tiles
 constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
-constexpr int NPACK = PB3 + 4 * 64;
+constexpr int PB1K = PB3 + 4 * 64;      // inference: K W1 (K = 2 log2 e, the tanh argument scale)
+constexpr int PB2K = PB1K + 4 * 64;     // inference: K W2
+constexpr int NPACK = PB2K + 16 * 64;
 
-typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfv2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
+// two values per v_cvt_pk_bf16_f32 (a 4-wide conversion is lowered one value
+// per instruction plus a permute)
+__device__ __forceinline__ unsigned cvt2(float a, float b)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, bfv2));
+}
+
 __device__ __forceinline__ bf4 cvt4(float a, float b, float c, float d)
 {
-    const f4 v = {a, b, c, d};
-    return __builtin_bit_cast(bf4, __builtin_convertvector(v, bfv4));
+    return __builtin_bit_cast(bf4, (u2){cvt2(a, b), cvt2(c, d)});
 }
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
@@ -68,6 +78,14 @@ __device__ __forceinline__ float act(float z, float kb)
 {
     const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(z, kTanhK, kb));
     return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(e + 1.f), 1.f);
+}
+
+// The same from the already scaled argument u = K (z + b): the inference
+// kernels multiply by K W (tiles PB1K, PB2K, rounded once from the FP32
+// weights) into accumulators that start at K b.
+__device__ __forceinline__ float act_scaled(float u)
+{
+    return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f), 1.f);
 }
 
 struct MlpW {
@@ -94,6 +112,7 @@ __device__ __forceinline__ bf4 lds_row4(const unsigned short* p)
 
 // Forward pass of 64 samples held in ws.X (BF16, already normalised);
 // leaves H1, H2 (BF16) in LDS and the outputs in ws.y.  Caller syncs.
+template <bool INFER>
 __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
 {
     const int r16 = lane & 15, q4 = lane >> 4;
@@ -101,27 +120,33 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
     for (int tm = 0; tm < 4; tm++) {
         const bf4 a = lds_row4(&ws.X[(tm * 16 + r16) * KP1 + 4 * q4]);
         for (int tn = 0; tn < 4; tn++) {
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma(a, w.pack[(PB1 >> 6) * 64 + tn * 64 + lane], acc);
             const float kb = kTanhK * w.master[OB1 + tn * 16 + r16];
+            f4 acc = INFER ? f4{kb, kb, kb, kb} : f4{0.f, 0.f, 0.f, 0.f};
+            acc = mfma(a, w.pack[(INFER ? PB1K : PB1) + tn * 64 + lane], acc);
 #pragma unroll
-            for (int e = 0; e < 4; e++) ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(act(acc[e], kb));
+            for (int e = 0; e < 4; e++)
+                ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(INFER ? act_scaled(acc[e]) : act(acc[e], kb));
         }
     }
     __syncthreads();
     // layer 2: H1 (64 x 64) W2 (64 x 64)
     for (int tm = 0; tm < 4; tm++) {
         f4 acc[4];
-        for (int tn = 0; tn < 4; tn++) acc[tn] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int tn = 0; tn < 4; tn++) {
+            const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
+            acc[tn] = INFER ? f4{kb, kb, kb, kb} : f4{0.f, 0.f, 0.f, 0.f};
+        }
         for (int ks = 0; ks < 4; ks++) {
             const bf4 a = lds_row4(&ws.H1[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
-            for (int tn = 0; tn < 4; tn++) acc[tn] = mfma(a, w.pack[PB2 / 64 * 64 + (ks * 4 + tn) * 64 + lane], acc[tn]);
+            for (int tn = 0; tn < 4; tn++)
+                acc[tn] = mfma(a, w.pack[(INFER ? PB2K : PB2) + (ks * 4 + tn) * 64 + lane], acc[tn]);
         }
         for (int tn = 0; tn < 4; tn++) {
             const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(act(acc[tn][e], kb));
+                ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
+                    f2bf(INFER ? act_scaled(acc[tn][e]) : act(acc[tn][e], kb));
         }
     }
     __syncthreads();
@@ -141,6 +166,35 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
 }
 
 // ---- features ----
+// Division of a 32-bit cell index n < 2^31 by a runtime divisor d: with
+// s = ceil(log2 d) and m = ceil(2^(31+s) / d) < 2^32, n / d =
+// (n m) >> (31 + s) exactly (the error of m / 2^(31+s) against 1/d stays
+// below 1/n); one 32 x 32 -> 64 multiply and a shift instead of an integer
+// division sequence.
+struct Div32 {
+    unsigned m;
+    int sh;
+};
+
+static Div32 div32(unsigned d)
+{
+    int s = 0;
+    while ((1ull << s) < d) s++;
+    const unsigned long long p = 1ull << (31 + s);
+    return Div32{(unsigned)((p + d - 1) / d), 31 + s};
+}
+
+__device__ __forceinline__ unsigned div_by(const Div32& q, unsigned n)
+{
+    return (unsigned)(((unsigned long long)n * q.m) >> q.sh);
+}
+
+struct CellDecode {
+    Div32 byD, byH;
+};
+
+static CellDecode cell_decode(const Geo& g) { return CellDecode{div32((unsigned)g.D), div32((unsigned)g.H)}; }
+
 struct FeatArgs {
     CState5 U;
     Geo g;
@@ -148,6 +202,7 @@ struct FeatArgs {
     double dU;        // velocity difference of the two streams
     const float* dw;  // vorticity thickness delta_w(x) per x index (L + 2)
     double dy;
+    CellDecode dec;   // interior index -> (i, j, k)
 };
 
 // Features of interior cell (i, j, k) from the 7-point stencil of u =
@@ -157,6 +212,9 @@ struct FeatArgs {
 // strain magnitude, src/turbulence.c:299-301).
 __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, int k, float f[NIN])
 {
+    // FP32 with FMA contraction and the hardware square root (1 ulp): the
+    // features reach the network rounded to BF16
+#pragma clang fp contract(fast)
     const Geo& g = a.g;
     const long long c = cidx(g, i, j, k);
     const long long st[3] = {g.si, g.sj, 1};
@@ -212,8 +270,8 @@ __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, 
             ssww += SS[r][e] * WW[e][r];
         }
     const float tiny = 1e-30f;
-    const float smag = sqrtf(2.f * s2);
-    const float sn = sqrtf(s2);
+    const float smag = __builtin_amdgcn_sqrtf(2.f * s2);
+    const float sn = __builtin_amdgcn_sqrtf(s2);
     const float om[3] = {gr[2][1] - gr[1][2], gr[0][2] - gr[2][0], gr[1][0] - gr[0][1]};
     float ws = 0.f, oo = 0.f;
 #pragma unroll
@@ -228,12 +286,22 @@ __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, 
     f[1] = s3 * __builtin_amdgcn_rcpf(s2 * sn + tiny);
     f[2] = sww * __builtin_amdgcn_rcpf(sn * w2 + tiny);
     f[3] = ssww * __builtin_amdgcn_rcpf(s2 * w2 + tiny);
-    const float delta = sqrtf((float)a.ph.DD);   // (dx dy dz)^(1/3)
-    f[4] = sqrtf(lap[0] * lap[0] + lap[1] * lap[1] + lap[2] * lap[2]) * delta * __builtin_amdgcn_rcpf(smag + tiny);
+    const float delta = __builtin_amdgcn_sqrtf((float)a.ph.DD);   // (dx dy dz)^(1/3)
+    const float lapn = __builtin_amdgcn_sqrtf(lap[0] * lap[0] + lap[1] * lap[1] + lap[2] * lap[2]);
+    f[4] = lapn * delta * __builtin_amdgcn_rcpf(smag + tiny);
     f[5] = ws * __builtin_amdgcn_rcpf(oo * smag + tiny);
     f[6] = smag * delta * (float)(1. / a.dU);
     f[7] = yv * __builtin_amdgcn_rcpf(a.dw[i]);
     return smag;
+}
+
+// interior cell t (k fastest) -> (i, j, k), 1-based
+__device__ __forceinline__ void decode_cell(const Geo& g, const CellDecode& c, unsigned t, int& i, int& j, int& k)
+{
+    const unsigned a = div_by(c.byD, t), b = div_by(c.byH, a);
+    k = (int)(t - a * (unsigned)g.D) + 1;
+    j = (int)(a - b * (unsigned)g.H) + 1;
+    i = (int)b + 1;
 }
 
 __device__ __forceinline__ void decode_interior(const Geo& g, long long t, int& i, int& j, int& k)
@@ -280,7 +348,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
     float kb1[4][4], kb2[4][4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        A1[t] = w.pack[PB1 + t * 64 + lane];
+        A1[t] = w.pack[PB1K + t * 64 + lane];
         A3[t] = w.pack[PB3 + t * 64 + lane];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -289,7 +357,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         }
     }
 #pragma unroll
-    for (int t = 0; t < 16; t++) A2[t] = w.pack[PB2 + t * 64 + lane];
+    for (int t = 0; t < 16; t++) A2[t] = w.pack[PB2K + t * 64 + lane];
     const float b3 = w.master[OB3];
     float nm[NIN], ns[NIN];
 #pragma unroll
@@ -306,7 +374,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         float f[NIN], smag = 0.f;
         int i = 1, j = 1, k = 1;
         if (valid) {
-            decode_interior(a.g, cell, i, j, k);
+            decode_cell(a.g, a.dec, (unsigned)cell, i, j, k);
             smag = cell_features(a, i, j, k, f);
         } else {
 #pragma unroll
@@ -333,19 +401,17 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
             for (int t = 0; t < 4; t++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    const f4 z = mfma(A1[t], xb[u], f4{0.f, 0.f, 0.f, 0.f});
-                    h1[u][t] = cvt4(act(z[0], kb1[t][0]), act(z[1], kb1[t][1]), act(z[2], kb1[t][2]),
-                                    act(z[3], kb1[t][3]));
+                    const f4 z = mfma(A1[t], xb[u], f4{kb1[t][0], kb1[t][1], kb1[t][2], kb1[t][3]});
+                    h1[u][t] = cvt4(act_scaled(z[0]), act_scaled(z[1]), act_scaled(z[2]), act_scaled(z[3]));
                 }
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    f4 z = {0.f, 0.f, 0.f, 0.f};
+                    f4 z = {kb2[t][0], kb2[t][1], kb2[t][2], kb2[t][3]};
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++) z = mfma(A2[ks * 4 + t], h1[u][ks], z);
-                    h2[u][t] = cvt4(act(z[0], kb2[t][0]), act(z[1], kb2[t][1]), act(z[2], kb2[t][2]),
-                                    act(z[3], kb2[t][3]));
+                    h2[u][t] = cvt4(act_scaled(z[0]), act_scaled(z[1]), act_scaled(z[2]), act_scaled(z[3]));
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
@@ -390,7 +456,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
-    forward64(ws, w, lane);
+    forward64<true>(ws, w, lane);
     if (s < n) y[s] = ws.y[lane];
 }
 
@@ -423,7 +489,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
-    forward64(ws, w, lane);
+    forward64<false>(ws, w, lane);
     const float wgt = T.wpow != 0.f ? powf(fmaxf(T.X[s * NIN + 6], 0.f), T.wpow) * T.winv : 1.f;
     const float err = ws.y[lane] - T.t[s];
     const float dy = wgt * err * (2.f / (float)T.batch);
@@ -563,9 +629,15 @@ __global__ void k_mlp_pack(const float* master, bf4* pack)
         } else if (tile < PB3 / 64) {               // W2^T: B[k][n] = W2[n][k]
             const int q = tile - PB2T / 64, ks = q / 4, tn = q % 4;
             x = master[OW2 + (tn * 16 + r16) * NH + ks * 16 + kk];
-        } else {                                    // w3 in column 0
+        } else if (tile < PB1K / 64) {              // w3 in column 0
             const int ks = tile - PB3 / 64;
             x = r16 == 0 ? master[OW3 + ks * 16 + kk] : 0.f;
+        } else if (tile < PB2K / 64) {              // K W1
+            const int n = (tile - PB1K / 64) * 16 + r16;
+            x = kk < NIN ? kTanhK * master[OW1 + kk * NH + n] : 0.f;
+        } else {                                    // K W2 [k][n]
+            const int q = tile - PB2K / 64, ks = q / 4, tn = q % 4;
+            x = kTanhK * master[OW2 + (ks * 16 + kk) * NH + tn * 16 + r16];
         }
         v[e] = x;
     }
@@ -734,6 +806,7 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     a.dU = dU;
     a.dw = dw;
     a.dy = dy;
+    a.dec = cell_decode(g);
     const long long N = (long long)g.L * g.H * g.D;
     const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
     // one resident wave of blocks (3 per CU at this kernel's occupancy) loops over the cells
@@ -763,6 +836,7 @@ void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph,
     a.dU = dU;
     a.dw = dw;
     a.dy = dy;
+    a.dec = cell_decode(g);
     hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, a, cd, stride,
                        offset, X, t, nout);
 }

@@ -31,14 +31,26 @@ def normalise(X, norm):
     return bf16((X - norm[:8]) * norm[8:])
 
 
-def forward(X, params, norm, exact=False):
+TANH_K = np.float32(2.8853900817779268)   # 2 log2(e): tanh(z) = 1 - 2 / (2^(K z) + 1)
+
+
+def forward(X, params, norm, exact=False, inference=False):
     """Network output for raw features X (n x 8).  exact=True skips every
-    BF16 rounding (FP32 network)."""
+    BF16 rounding (FP32 network).  inference=True follows the inference
+    kernels' rounding: hidden layers multiply by BF16(K W) (K W formed in
+    FP32) onto K b, and tanh takes that sum divided by K; training uses
+    BF16(W) and b."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
     r = (lambda a: np.asarray(a, np.float32)) if exact else bf16
+
+    def layer(h, W, b):
+        if inference and not exact:
+            return np.tanh((h.astype(np.float64) @ r(TANH_K * W) + TANH_K * b) / TANH_K)
+        return np.tanh(h.astype(np.float64) @ r(W) + b)
+
     x = r((np.asarray(X, np.float32) - norm[:8]) * norm[8:])
-    h1 = r(np.tanh(x.astype(np.float64) @ r(W1) + b1).astype(np.float32))
-    h2 = r(np.tanh(h1.astype(np.float64) @ r(W2) + b2).astype(np.float32))
+    h1 = r(layer(x, W1, b1).astype(np.float32))
+    h2 = r(layer(h1, W2, b2).astype(np.float32))
     y = h2.astype(np.float64) @ r(w3) + b3
     return y.astype(np.float32), (x, h1, h2)
 

@@ -58,7 +58,7 @@ def test_forward_matches_numpy():
         y = ctx.mlp_forward(X)
         q, m = ctx.mlp_get_params()
     assert np.array_equal(q, p) and np.array_equal(m, norm)
-    yr, _ = mlp_ref.forward(X, p, norm)
+    yr, _ = mlp_ref.forward(X, p, norm, inference=True)
     ye, _ = mlp_ref.forward(X, p, norm, exact=True)
     d = np.abs(y - yr)
     print("forward: max |y - y_bf16| %.2e, mean %.2e, max |y - y_fp32| %.2e"
