@@ -12,6 +12,20 @@
 
 namespace fsl {
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a workgroup
+// fence over global memory as well, which makes every wave wait for all of
+// its outstanding global loads (s_waitcnt vmcnt(0)) at each barrier, so
+// loads issued ahead for the next plane would complete there.  The stage
+// kernels exchange data between threads through LDS only (their global
+// inputs and outputs are distinct arrays), so an LDS-scoped fence suffices:
+// s_waitcnt lgkmcnt(0) + s_barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // src/def.h:30-49
 constexpr double kK = 1.4;
 constexpr double kK_1 = 0.4;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
         // its use after the last barrier would otherwise wait for the load
         double bq[5];
         if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si + col, bq);
-        __syncthreads();   // the previous plane's LDS readers are done
+        lds_barrier();   // the previous plane's LDS readers are done
         const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {
             double vr[3];
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
         }
         double xp[5], xm[5];
         recon_dir<1, 2, 3>(P, w2, w0, false, xp, xm);
-        __syncthreads();
+        lds_barrier();
 
         // ---- x face i - 1/2 and the update of cell i - 1 ----
         if (i >= i0 && interior) {
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (inseg && interior) {
 #pragma unroll
             for (int f = 0; f < 5; f++) {

@@ -147,7 +147,16 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     for (int i = i0 - 1; i <= i1; i++) {
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
+        // Loads in the order of first use (the wait counter retires in issue
+        // order): next plane's cell and this cell's mu_SGS (end of the
+        // reconstruction), the step state of the updated cell (after the
+        // first barrier), the next plane's y/z neighbours (next iteration).
         load5(a.in, (long long)min(i + 1, L + 1) * si, pb, colb, w2);
+        // own cell's mu_SGS; the faces take theirs from the LDS ring
+        const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colb) : 0.;
+        // step state of the cell this iteration updates (i - 1)
+        double bq[5];
+        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
         // y and z neighbours of this plane's cell (loaded one iteration ahead)
         double nyp[5], nym[5], nzp[5], nzm[5];
 #pragma unroll
@@ -159,12 +168,6 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             load5(a.in, pn, pb, ozp, pzp);
             load5(a.in, pn, pb, ozm, pzm);
         }
-        // step state of the cell this iteration updates (i - 1), issued early:
-        // its use after the last barrier would otherwise wait for the load
-        double bq[5];
-        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
-        // own cell's mu_SGS; the faces take theirs from the LDS ring
-        const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colb) : 0.;
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
         // minus states) was last read by the previous plane's face loops,
@@ -174,26 +177,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const int vc = i & 1, vb = vc ^ 1;
         mx::PrimF P;
         mx::cell_prims(w1, ri1, v1, P);
-        {
-            // 1/rho and velocity of plane i + 1: LDS ring, and this cell's next iteration
-            const float ri = mx::rcp((float)w2[0]);
-            const int r = ring(i + 1);
-            float v2[3];
-#pragma unroll
-            for (int e = 0; e < 3; e++) {
-                v2[e] = (float)w2[1 + e] * ri;
-                s_vr[r][e][c] = v2[e];
-            }
-            s_visc[vc][0][c] = v1[0];
-            s_visc[vc][1][c] = v1[1];
-            s_visc[vc][2][c] = v1[2];
-            s_visc[vc][3][c] = mx::cell_temp(P);
-            s_visc[vc][4][c] = (float)w1[0];
-            s_mu[vc][c] = mu_c;
-            ri1 = ri;
-#pragma unroll
-            for (int e = 0; e < 3; e++) v1[e] = v2[e];
-        }
+        s_visc[vc][0][c] = v1[0];
+        s_visc[vc][1][c] = v1[1];
+        s_visc[vc][2][c] = v1[2];
+        s_visc[vc][3][c] = mx::cell_temp(P);
+        s_visc[vc][4][c] = (float)w1[0];
+        // the y/z reconstruction needs nothing loaded in this iteration; the
+        // uses of w2 and mu_c follow it, so that it covers their latency
         const float qf[4] = {(float)w1[0], (float)w1[1], (float)w1[2], (float)w1[3]};
         const bool inseg = (i >= i0 && i < i1);
         mx::FaceF yp, zp;
@@ -218,7 +208,19 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
             for (int f = 0; f < 5; f++) dxp[f] = dp[f];
         }
-        __syncthreads();
+        {
+            // 1/rho and velocity of plane i + 1: LDS ring, and this cell's next iteration
+            const float ri = mx::rcp((float)w2[0]);
+            const int r = ring(i + 1);
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                v1[e] = (float)w2[1 + e] * ri;
+                s_vr[r][e][c] = v1[e];
+            }
+            ri1 = ri;
+            s_mu[vc][c] = mu_c;
+        }
+        lds_barrier();
 
         if (i >= i0 && interior) {
             mx::FaceF l, r;
@@ -348,7 +350,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (inseg && interior) {
 #pragma unroll
             for (int f = 0; f < 5; f++) {
