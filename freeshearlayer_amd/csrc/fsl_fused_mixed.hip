@@ -400,13 +400,10 @@ void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* o
     a.dtZ = v.dtZ;
     a.cfl_out = stage == 3 ? v.red : nullptr;
     a.cfl_mode = v.cfl_mode;
-    static int minw = -1;
-    if (minw < 0) {
-        const char* e = getenv("FSL_MIXED_MINW");
-        minw = (e && atoi(e) == 3) ? 3 : 2;
-    }
-    if (minw == 3) launch_mixed<16, 16, 3>(v.stream, stage, a, 3);
-    else launch_mixed<16, 16, 2>(v.stream, stage, a, 2);
+    // 16 x 16 workgroups at two waves per SIMD (two workgroups per CU, whose
+    // barriers overlap); 32 x 16 and 16 x 32 tiles (one workgroup per CU)
+    // and three waves per SIMD (register spills) measured slower
+    launch_mixed<16, 16, 2>(v.stream, stage, a, 2);
 }
 
 }  // namespace fsl
