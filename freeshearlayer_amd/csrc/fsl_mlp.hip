@@ -88,6 +88,17 @@ __device__ __forceinline__ float act_scaled(float u)
     return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f), 1.f);
 }
 
+// Each wavefront of the inference kernel stages its own 64 cells' features in
+// its own LDS slice and reads them back across its lanes: ordering the
+// wave's LDS accesses suffices (s_waitcnt lgkmcnt(0)); no workgroup barrier,
+// so the block's waves run independently.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 struct MlpW {
     const bf4* pack;      // NPACK / 64 operand tiles, bf4 per lane
     const float* master;  // biases (and w3 for the backward pass)
@@ -385,7 +396,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         for (int q = 0; q < NIN; q++) fn[q] = (f[q] - nm[q]) * ns[q];
         *(bf4*)&X[lane * NIN] = cvt4(fn[0], fn[1], fn[2], fn[3]);
         *(bf4*)&X[lane * NIN + 4] = cvt4(fn[4], fn[5], fn[6], fn[7]);
-        __syncthreads();
+        wave_lds_sync();
         float ymine = 0.f;
 #pragma unroll
         for (int s0 = 0; s0 < 4; s0 += TP) {
@@ -429,7 +440,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
             mu[c] = a.U.f[0][c] * (double)Cd * a.ph.DD * (double)smag;
             if (cd_out) cd_out[c] = Cd;
         }
-        __syncthreads();   // X is rewritten by the next iteration
+        wave_lds_sync();   // X is rewritten by the next iteration
     }
 }
 
