@@ -76,6 +76,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
     __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
+    __shared__ double s_w[2][5][NT];    // state of planes i, i+1 (slot p & 1): y/z neighbours
 
     const Geo& g = a.g;
     const int L = g.L, H = g.H, D = g.D;
@@ -109,6 +110,14 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const unsigned colb = (unsigned)jc * sjb + (unsigned)kc * 8u;
     const unsigned oyp = (unsigned)jp * sjb + (unsigned)kc * 8u, oym = (unsigned)jm * sjb + (unsigned)kc * 8u;
     const unsigned ozp = (unsigned)jc * sjb + (unsigned)zkp * 8u, ozm = (unsigned)jc * sjb + (unsigned)zkm * 8u;
+    // Neighbours held by another lane of the workgroup (its own cell of the
+    // plane, shared through s_w): all but the tile's edge rows / columns and
+    // the periodic ghost columns (kc = 0, D + 1), which load theirs.  Lanes
+    // clamped beyond the domain (j > H + 1, k > D + 1) may take a clamped
+    // neighbour's state: their face states are not used.
+    const bool zin = kc >= 1 && kc <= D;
+    const bool ypin = ty < BJ - 1, ymin = ty > 0, zpin = tx < BK - 1 && zin, zmin = tx > 0 && zin;
+    const int cyp = ypin ? c + BK : c, cym = ymin ? c - BK : c, czp = zpin ? c + 1 : c, czm = zmin ? c - 1 : c;
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
@@ -129,10 +138,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
         load5(a.in, (long long)ia * si, pb, colb, w1);
         load5(a.in, (long long)ib * si, pb, colb, w2);
-        load5(a.in, (long long)ib * si, pb, oyp, pyp);
-        load5(a.in, (long long)ib * si, pb, oym, pym);
-        load5(a.in, (long long)ib * si, pb, ozp, pzp);
-        load5(a.in, (long long)ib * si, pb, ozm, pzm);
+#pragma unroll
+        for (int f = 0; f < 5; f++) { pyp[f] = 0.; pym[f] = 0.; pzp[f] = 0.; pzm[f] = 0.; }
 #pragma unroll
         for (int f = 0; f < 5; f++) dxp[f] = (float)(w2[f] - w1[f]);
         ri1 = mx::rcp((float)w2[0]);
@@ -157,16 +164,24 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // step state of the cell this iteration updates (i - 1)
         double bq[5];
         if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
-        // y and z neighbours of this plane's cell (loaded one iteration ahead)
+        // y and z neighbours of this plane's cell: from s_w (written by their
+        // lanes in the previous iteration) or, on the edge lanes, loaded one
+        // iteration ahead
         double nyp[5], nym[5], nzp[5], nzm[5];
-#pragma unroll
-        for (int f = 0; f < 5; f++) { nyp[f] = pyp[f]; nym[f] = pym[f]; nzp[f] = pzp[f]; nzm[f] = pzm[f]; }
         {
+            const int sl = i & 1;
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                nyp[f] = ypin ? s_w[sl][f][cyp] : pyp[f];
+                nym[f] = ymin ? s_w[sl][f][cym] : pym[f];
+                nzp[f] = zpin ? s_w[sl][f][czp] : pzp[f];
+                nzm[f] = zmin ? s_w[sl][f][czm] : pzm[f];
+            }
             const long long pn = (long long)min(i + 1, L + 1) * si;
-            load5(a.in, pn, pb, oyp, pyp);
-            load5(a.in, pn, pb, oym, pym);
-            load5(a.in, pn, pb, ozp, pzp);
-            load5(a.in, pn, pb, ozm, pzm);
+            if (!ypin) load5(a.in, pn, pb, oyp, pyp);
+            if (!ymin) load5(a.in, pn, pb, oym, pym);
+            if (!zpin) load5(a.in, pn, pb, ozp, pzp);
+            if (!zmin) load5(a.in, pn, pb, ozm, pzm);
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
@@ -219,6 +234,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             }
             ri1 = ri;
             s_mu[vc][c] = mu_c;
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][c] = w2[f];
         }
         lds_barrier();
 
