@@ -280,7 +280,8 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         }
         if (hipMalloc(&c->mu, c->g.G * sizeof(double)) != hipSuccess ||
             hipMalloc(&c->u_in, (size_t)H * sizeof(double)) != hipSuccess ||
-            hipMalloc(&c->red, 64) != hipSuccess) {
+            hipMalloc(&c->red, 64) != hipSuccess ||
+            hipHostMalloc((void**)&c->red_host, 64, hipHostMallocDefault) != hipSuccess) {
             rc = set_err(c, FSL_ERR_NOMEM, "allocation failed");
             break;
         }
@@ -337,6 +338,7 @@ void fsl_destroy(fsl_ctx* c)
     (void)hipFree(c->u_in);
     (void)hipFree(c->sums);
     (void)hipFree(c->red);
+    if (c->red_host) (void)hipHostFree(c->red_host);
     learned_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -617,6 +619,7 @@ int fsl_check_co_num(fsl_ctx* c, double* maxCo_out)
     if (!c->cfl_ready) {
         // otherwise the last fused stage 3 reduced the same quantity while writing U
         HIPCHK(c, hipMemsetAsync(c->red, 0, sizeof(unsigned long long), c->stream));
+        c->cfl_zeroed = 0;   // the cell now holds this reduction
         prof_mark(c, K_CFL, 0);
         launch_cfl(c->stream, mode, cst5(c->U, c->g.G), c->g, c->ph, c->dtX, c->red);
         prof_mark(c, K_CFL, 1);
@@ -624,9 +627,11 @@ int fsl_check_co_num(fsl_ctx* c, double* maxCo_out)
         if (rc) return rc;
     }
     c->cfl_ready = 0;
-    unsigned long long bits = 0;
-    HIPCHK(c, hipMemcpyAsync(&bits, c->red, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    // the maximum comes back through pinned memory (a pageable destination
+    // is staged by the runtime and costs tens of microseconds per step)
+    HIPCHK(c, hipMemcpyAsync(c->red_host, c->red, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned long long bits = *c->red_host;
     double m;
     memcpy(&m, &bits, sizeof m);
     if (maxCo_out) *maxCo_out = m;
@@ -774,9 +779,12 @@ int ctx_check(fsl_ctx* c, const char* what) { return check_launch(c, what); }
 int ctx_cfl_reset(fsl_ctx* c)
 {
     c->cfl_ready = 0;
-    HIPCHK(c, hipMemsetAsync(c->red, 0, sizeof(unsigned long long), c->stream));
+    // a fused stage 2 on the same stream has already cleared the cell
+    if (!c->cfl_zeroed) HIPCHK(c, hipMemsetAsync(c->red, 0, sizeof(unsigned long long), c->stream));
+    c->cfl_zeroed = 0;
     return FSL_OK;
 }
+void ctx_cfl_zeroed(fsl_ctx* c, int z) { c->cfl_zeroed = z; }
 void ctx_cfl_ready(fsl_ctx* c) { c->cfl_ready = 1; }
 int ctx_sgs(fsl_ctx* c) { return fsl_dynamic_smagorinsky(c); }
 void ctx_set_cur(fsl_ctx* c, int is_p) { c->cur_is_p = is_p; }

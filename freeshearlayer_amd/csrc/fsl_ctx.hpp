@@ -69,6 +69,7 @@ struct fsl_ctx {
     double* u_in;       // HIG
     double* sums;       // partials + 5 results
     unsigned long long* red;
+    unsigned long long* red_host;   // pinned copy of the CFL maximum (hipHostMalloc)
     size_t bytes;
 
     int cur_is_p;       // stage input: U (0) or Up (1)
@@ -76,6 +77,7 @@ struct fsl_ctx {
     fsl_block blk;
     unsigned step;
     int cfl_ready;      // c->red holds the CFL maximum of U (from fused stage 3) for dtX
+    int cfl_zeroed;     // a fused stage 2 has cleared c->red for the next stage 3
 
     int prof;
     int prof_open_id;

@@ -33,6 +33,7 @@ int ctx_sgs(fsl_ctx* c);
 int learned_sgs_step(fsl_ctx* c);
 int ctx_cfl_reset(fsl_ctx* c);
 void ctx_cfl_ready(fsl_ctx* c);
+void ctx_cfl_zeroed(fsl_ctx* c, int z);
 int ctx_ghost(fsl_ctx* c, const double* state);
 void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
@@ -50,6 +51,7 @@ struct FusedArgs {
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
     unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (or null)
+    unsigned long long* cfl_zero;  // stage 2: CFL cell to clear for the following stage 3 (or null)
     int cfl_mode;
 };
 
@@ -84,6 +86,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
         const int per = gridDim.x >> 3;
         bid = (bid & 7) * per + (bid >> 3);
     }
+    // stage 2 clears the CFL cell that the following stage 3 reduces into
+    // (one vector store; the stream orders it before stage 3)
+    if (STAGE == 2 && a.cfl_zero && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.cfl_zero = 0ull;
     if (bid >= a.nblocks) return;
     const int tk = bid % a.ntk;
     const int rem = bid / a.ntk;
@@ -452,6 +457,7 @@ int fused_stage(fsl_ctx* ctx, int stage)
     // stage 3 also reduces the CFL quantity of the new state for the next
     // step's time step (fsl_check_co_num uses it while U and dt are unchanged)
     a.cfl_out = stage == 3 ? v.red : nullptr;
+    a.cfl_zero = stage == 2 ? v.red : nullptr;
     a.cfl_mode = v.cfl_mode;
     if (stage == 3 && (rc = ctx_cfl_reset(ctx))) return rc;
     ctx_prof(ctx, K_FUSED_XYZ, 0);
@@ -467,6 +473,7 @@ int fused_stage(fsl_ctx* ctx, int stage)
     rc = ctx_check(ctx, "fused stage");
     if (rc) return rc;
     if (stage == 3) ctx_cfl_ready(ctx);
+    if (stage == 2) ctx_cfl_zeroed(ctx, 1);
     // stage inputs: U, then Up, then Uq; the state after stage 3 is U
     ctx_set_cur(ctx, stage == 3 ? 0 : 1);
     return FSL_OK;

@@ -29,6 +29,7 @@ struct MixedArgs {
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
     unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (FP64; or null)
+    unsigned long long* cfl_zero;  // stage 2: CFL cell to clear for the following stage 3 (or null)
     int cfl_mode;
 };
 
@@ -85,6 +86,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const int per = gridDim.x >> 3;
         bid = (bid & 7) * per + (bid >> 3);
     }
+    // stage 2 clears the CFL cell that the following stage 3 reduces into
+    // (one vector store; the stream orders it before stage 3)
+    if (STAGE == 2 && a.cfl_zero && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.cfl_zero = 0ull;
     if (bid >= a.nblocks) return;
     const int tk = bid % a.ntk;
     const int rem = bid / a.ntk;
@@ -416,6 +420,7 @@ void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* o
     a.dtY = v.dtY;
     a.dtZ = v.dtZ;
     a.cfl_out = stage == 3 ? v.red : nullptr;
+    a.cfl_zero = stage == 2 ? v.red : nullptr;
     a.cfl_mode = v.cfl_mode;
     // 16 x 16 workgroups at two waves per SIMD (two workgroups per CU, whose
     // barriers overlap); 32 x 16 and 16 x 32 tiles (one workgroup per CU)
