@@ -122,13 +122,19 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const bool zin = kc >= 1 && kc <= D;
     const bool ypin = ty < BJ - 1, ymin = ty > 0, zpin = tx < BK - 1 && zin, zmin = tx > 0 && zin;
     const int cyp = ypin ? c + BK : c, cym = ymin ? c - BK : c, czp = zpin ? c + 1 : c, czm = zmin ? c - 1 : c;
+    // Edge lanes load one neighbour per direction (a lane lacks at most one y
+    // side and, unless it is a periodic ghost column, one z side), so each
+    // direction needs one load set with a per-lane offset; ghost columns load
+    // their second z neighbour into a third set.
+    const bool yload = !ypin || !ymin, zload = !zpin || !zmin, z2load = !zpin && !zmin;
+    const unsigned oye = ymin ? oyp : oym, oze = zmin ? ozp : ozm;
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
-    double pyp[5], pym[5], pzp[5], pzm[5];   // y/z neighbours of the cell of the next plane
+    double pye[5], pze[5], pz2[5];           // edge lanes: y/z neighbours of the next plane's cell
     float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
     float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
     mx::FaceF xp_prev;
@@ -143,7 +149,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         load5(a.in, (long long)ia * si, pb, colb, w1);
         load5(a.in, (long long)ib * si, pb, colb, w2);
 #pragma unroll
-        for (int f = 0; f < 5; f++) { pyp[f] = 0.; pym[f] = 0.; pzp[f] = 0.; pzm[f] = 0.; }
+        for (int f = 0; f < 5; f++) { pye[f] = 0.; pze[f] = 0.; pz2[f] = 0.; }
 #pragma unroll
         for (int f = 0; f < 5; f++) dxp[f] = (float)(w2[f] - w1[f]);
         ri1 = mx::rcp((float)w2[0]);
@@ -176,16 +182,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             const int sl = i & 1;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
-                nyp[f] = ypin ? s_w[sl][f][cyp] : pyp[f];
-                nym[f] = ymin ? s_w[sl][f][cym] : pym[f];
-                nzp[f] = zpin ? s_w[sl][f][czp] : pzp[f];
-                nzm[f] = zmin ? s_w[sl][f][czm] : pzm[f];
+                nyp[f] = ypin ? s_w[sl][f][cyp] : pye[f];
+                nym[f] = ymin ? s_w[sl][f][cym] : pye[f];
+                nzp[f] = zpin ? s_w[sl][f][czp] : (zmin ? pze[f] : pz2[f]);
+                nzm[f] = zmin ? s_w[sl][f][czm] : pze[f];
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
-            if (!ypin) load5(a.in, pn, pb, oyp, pyp);
-            if (!ymin) load5(a.in, pn, pb, oym, pym);
-            if (!zpin) load5(a.in, pn, pb, ozp, pzp);
-            if (!zmin) load5(a.in, pn, pb, ozm, pzm);
+            if (yload) load5(a.in, pn, pb, oye, pye);
+            if (zload) load5(a.in, pn, pb, oze, pze);
+            if (z2load) load5(a.in, pn, pb, ozp, pz2);
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
