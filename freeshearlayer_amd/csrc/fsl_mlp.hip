@@ -42,9 +42,13 @@ constexpr int PB1 = 0;                  // W1: 4 n-tiles
 constexpr int PB2 = PB1 + 4 * 64;       // W2: 4 k-steps x 4 n-tiles
 constexpr int PB2T = PB2 + 16 * 64;     // W2 transposed (backward): 4 k-steps x 4 n-tiles
 constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
-constexpr int PB1K = PB3 + 4 * 64;      // inference: K W1 (K = 2 log2 e, the tanh argument scale)
-constexpr int PB2K = PB1K + 4 * 64;     // inference: K W2
-constexpr int NPACK = PB2K + 16 * 64;
+// FP16 operands of the inference kernels (K = 2 log2 e, the tanh argument
+// scale; the hidden layers carry r = 1 / (2^u + 1), tanh = 1 - 2 r):
+constexpr int PH1 = PB3 + 4 * 64;       // K W1
+constexpr int PH2 = PH1 + 4 * 64;       // -2 K W2
+constexpr int PH3 = PH2 + 16 * 64;      // -2 w3 in column 0
+constexpr int PHF = PH3 + 4 * 64;       // FP32: K (b2 + sum_k W2[k][n]) (64), b3 + sum_k w3[k] (1)
+constexpr int NPACK = PHF + 64;
 
 typedef __bf16 bfv2 __attribute__((ext_vector_type(2)));
 typedef float f2 __attribute__((ext_vector_type(2)));
@@ -53,21 +57,32 @@ typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
-// two values per v_cvt_pk_bf16_f32 (a 4-wide conversion is lowered one value
-// per instruction plus a permute)
-__device__ __forceinline__ unsigned cvt2(float a, float b)
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> FP16, round to nearest even, two values per v_cvt_pk_f16_f32
+__device__ __forceinline__ unsigned cvt2h(float a, float b)
 {
-    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, bfv2));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, h2));
 }
 
-__device__ __forceinline__ bf4 cvt4(float a, float b, float c, float d)
+__device__ __forceinline__ h4 cvt4h(float a, float b, float c, float d)
 {
-    return __builtin_bit_cast(bf4, (u2){cvt2(a, b), cvt2(c, d)});
+    return __builtin_bit_cast(h4, (u2){cvt2h(a, b), cvt2h(c, d)});
 }
+
+__device__ __forceinline__ unsigned short f2h(float f) { return __builtin_bit_cast(unsigned short, (_Float16)f); }
+
+// normalised features enter the FP16 products clamped to the FP16 range
+// (beyond a few units the first layer is saturated anyway)
+constexpr float kF16Max = 60000.f;
+__device__ __forceinline__ float clamp_h(float x) { return __builtin_amdgcn_fmed3f(x, -kF16Max, kF16Max); }
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ h4 as_h4(bf4 v) { return __builtin_bit_cast(h4, v); }
 
 // tanh(z + b) = 1 - 2 / (2^(K (z + b)) + 1), K = 2 log2(e), with kb = K b:
 // one FMA for the scaled argument, v_exp_f32, v_rcp_f32 and one FMA.  No
@@ -80,12 +95,15 @@ __device__ __forceinline__ float act(float z, float kb)
     return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(e + 1.f), 1.f);
 }
 
-// The same from the already scaled argument u = K (z + b): the inference
-// kernels multiply by K W (tiles PB1K, PB2K, rounded once from the FP32
-// weights) into accumulators that start at K b.
-__device__ __forceinline__ float act_scaled(float u)
+// Inference activation r = 1 / (2^u + 1) of the scaled argument
+// u = K (z + b) (tanh(z + b) = 1 - 2 r): v_exp_f32, an add, v_rcp_f32.  The
+// inference kernels multiply FP16 operands: the features, K W1 onto K b1;
+// r1 and -2 K W2 onto K (b2 + sum_k W2[k][.]); r2 and -2 w3 onto
+// b3 + sum_k w3[k] -- the network's products with h = 1 - 2 r.  FP16 keeps
+// r to 2^-12 (h to 5e-4), finer than the BF16 h of training (2e-3 near 1).
+__device__ __forceinline__ float act_r(float u)
 {
-    return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f), 1.f);
+    return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f);
 }
 
 // Each wavefront of the inference kernel stages its own 64 cells' features in
@@ -121,22 +139,26 @@ __device__ __forceinline__ bf4 lds_row4(const unsigned short* p)
     return v;
 }
 
-// Forward pass of 64 samples held in ws.X (BF16, already normalised);
-// leaves H1, H2 (BF16) in LDS and the outputs in ws.y.  Caller syncs.
+// Forward pass of 64 samples held in ws.X (already normalised; BF16, FP16
+// for INFER); leaves H1, H2 in LDS (BF16 h, FP16 r for INFER) and the
+// outputs in ws.y.  Training (INFER false) uses the BF16 tiles and tanh; the
+// inference form matches k_mlp_mu_tr's arithmetic.  Caller syncs.
 template <bool INFER>
 __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
 {
     const int r16 = lane & 15, q4 = lane >> 4;
+    const float* cf = reinterpret_cast<const float*>(w.pack + PHF);
     // layer 1: X (64 x 16) W1 (16 x 64)
     for (int tm = 0; tm < 4; tm++) {
         const bf4 a = lds_row4(&ws.X[(tm * 16 + r16) * KP1 + 4 * q4]);
         for (int tn = 0; tn < 4; tn++) {
             const float kb = kTanhK * w.master[OB1 + tn * 16 + r16];
-            f4 acc = INFER ? f4{kb, kb, kb, kb} : f4{0.f, 0.f, 0.f, 0.f};
-            acc = mfma(a, w.pack[(INFER ? PB1K : PB1) + tn * 64 + lane], acc);
+            f4 acc;
+            if (INFER) acc = mfma_h(as_h4(a), as_h4(w.pack[PH1 + tn * 64 + lane]), f4{kb, kb, kb, kb});
+            else acc = mfma(a, w.pack[PB1 + tn * 64 + lane], f4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = f2bf(INFER ? act_scaled(acc[e]) : act(acc[e], kb));
+                ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = INFER ? f2h(act_r(acc[e])) : f2bf(act(acc[e], kb));
         }
     }
     __syncthreads();
@@ -144,20 +166,22 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
     for (int tm = 0; tm < 4; tm++) {
         f4 acc[4];
         for (int tn = 0; tn < 4; tn++) {
-            const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
-            acc[tn] = INFER ? f4{kb, kb, kb, kb} : f4{0.f, 0.f, 0.f, 0.f};
+            const float c2 = cf[tn * 16 + r16];
+            acc[tn] = INFER ? f4{c2, c2, c2, c2} : f4{0.f, 0.f, 0.f, 0.f};
         }
         for (int ks = 0; ks < 4; ks++) {
             const bf4 a = lds_row4(&ws.H1[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
-            for (int tn = 0; tn < 4; tn++)
-                acc[tn] = mfma(a, w.pack[(INFER ? PB2K : PB2) + (ks * 4 + tn) * 64 + lane], acc[tn]);
+            for (int tn = 0; tn < 4; tn++) {
+                if (INFER) acc[tn] = mfma_h(as_h4(a), as_h4(w.pack[PH2 + (ks * 4 + tn) * 64 + lane]), acc[tn]);
+                else acc[tn] = mfma(a, w.pack[PB2 + (ks * 4 + tn) * 64 + lane], acc[tn]);
+            }
         }
         for (int tn = 0; tn < 4; tn++) {
             const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
-                    f2bf(INFER ? act_scaled(acc[tn][e]) : act(acc[tn][e], kb));
+                    INFER ? f2h(act_r(acc[tn][e])) : f2bf(act(acc[tn][e], kb));
         }
     }
     __syncthreads();
@@ -166,11 +190,13 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         f4 acc = {0.f, 0.f, 0.f, 0.f};
         for (int ks = 0; ks < 4; ks++) {
             const bf4 a = lds_row4(&ws.H2[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
-            acc = mfma(a, w.pack[PB3 / 64 * 64 + ks * 64 + lane], acc);
+            if (INFER) acc = mfma_h(as_h4(a), as_h4(w.pack[PH3 + ks * 64 + lane]), acc);
+            else acc = mfma(a, w.pack[PB3 + ks * 64 + lane], acc);
         }
         if (r16 == 0) {
+            const float b3 = INFER ? cf[64] : w.master[OB3];
 #pragma unroll
-            for (int e = 0; e < 4; e++) ws.y[tm * 16 + 4 * q4 + e] = acc[e] + w.master[OB3];
+            for (int e = 0; e < 4; e++) ws.y[tm * 16 + 4 * q4 + e] = acc[e] + b3;
         }
     }
     __syncthreads();
@@ -355,21 +381,22 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
     __shared__ unsigned short sX[4][64 * NIN];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q4 = lane >> 4;
-    bf4 A1[4], A2[16], A3[4];
+    h4 A1[4], A2[16], A3[4];
     float kb1[4][4], kb2[4][4];
+    const float* cf = reinterpret_cast<const float*>(w.pack + PHF);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        A1[t] = w.pack[PB1K + t * 64 + lane];
-        A3[t] = w.pack[PB3 + t * 64 + lane];
+        A1[t] = as_h4(w.pack[PH1 + t * 64 + lane]);
+        A3[t] = as_h4(w.pack[PH3 + t * 64 + lane]);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             kb1[t][e] = kTanhK * w.master[OB1 + t * 16 + 4 * q4 + e];
-            kb2[t][e] = kTanhK * w.master[OB2 + t * 16 + 4 * q4 + e];
+            kb2[t][e] = cf[t * 16 + 4 * q4 + e];
         }
     }
 #pragma unroll
-    for (int t = 0; t < 16; t++) A2[t] = w.pack[PB2K + t * 64 + lane];
-    const float b3 = w.master[OB3];
+    for (int t = 0; t < 16; t++) A2[t] = as_h4(w.pack[PH2 + t * 64 + lane]);
+    const float b3 = cf[64];
     float nm[NIN], ns[NIN];
 #pragma unroll
     for (int q = 0; q < NIN; q++) {
@@ -379,7 +406,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
     const long long N = (long long)a.g.L * a.g.H * a.g.D;
     const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
     unsigned short* X = sX[wv];
-    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs
+    // (workgroup b on XCD b % 8), so XCD x walks its own contiguous range of
+    // cell blocks in order and the j +- 1 and i +- 1 rows of the stencil come
+    // from its own L2.  The grid is a multiple of 8.
+    const long long per = gridDim.x >> 3, nb8 = (nblk + 7) >> 3;
+    const long long first = (long long)(blockIdx.x & 7) * nb8;
+    for (long long t = blockIdx.x >> 3; t < nb8; t += per) {
+        const long long blk = first + t;
+        if (blk >= nblk) break;   // uniform over the workgroup
         const long long cell = blk * MU_CELLS + threadIdx.x;
         const bool valid = cell < N;
         float f[NIN], smag = 0.f;
@@ -393,27 +428,27 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         }
         float fn[NIN];
 #pragma unroll
-        for (int q = 0; q < NIN; q++) fn[q] = (f[q] - nm[q]) * ns[q];
-        *(bf4*)&X[lane * NIN] = cvt4(fn[0], fn[1], fn[2], fn[3]);
-        *(bf4*)&X[lane * NIN + 4] = cvt4(fn[4], fn[5], fn[6], fn[7]);
+        for (int q = 0; q < NIN; q++) fn[q] = clamp_h((f[q] - nm[q]) * ns[q]);
+        *(h4*)&X[lane * NIN] = cvt4h(fn[0], fn[1], fn[2], fn[3]);
+        *(h4*)&X[lane * NIN + 4] = cvt4h(fn[4], fn[5], fn[6], fn[7]);
         wave_lds_sync();
         float ymine = 0.f;
 #pragma unroll
         for (int s0 = 0; s0 < 4; s0 += TP) {
             // B operand of layer 1: features 4 q4 .. 4 q4 + 3 of cell st * 16 + r16 (K padded to 16)
-            bf4 xb[TP];
+            h4 xb[TP];
 #pragma unroll
             for (int u = 0; u < TP; u++) {
-                xb[u] = bf4{0, 0, 0, 0};
-                if (q4 < 2) xb[u] = *(const bf4*)&X[((s0 + u) * 16 + r16) * NIN + 4 * q4];
+                xb[u] = h4{0, 0, 0, 0};
+                if (q4 < 2) xb[u] = *(const h4*)&X[((s0 + u) * 16 + r16) * NIN + 4 * q4];
             }
-            bf4 h1[TP][4], h2[TP][4];
+            h4 h1[TP][4], h2[TP][4];
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    const f4 z = mfma(A1[t], xb[u], f4{kb1[t][0], kb1[t][1], kb1[t][2], kb1[t][3]});
-                    h1[u][t] = cvt4(act_scaled(z[0]), act_scaled(z[1]), act_scaled(z[2]), act_scaled(z[3]));
+                    const f4 z = mfma_h(A1[t], xb[u], f4{kb1[t][0], kb1[t][1], kb1[t][2], kb1[t][3]});
+                    h1[u][t] = cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
                 }
 #pragma unroll
             for (int t = 0; t < 4; t++)
@@ -421,14 +456,14 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
                 for (int u = 0; u < TP; u++) {
                     f4 z = {kb2[t][0], kb2[t][1], kb2[t][2], kb2[t][3]};
 #pragma unroll
-                    for (int ks = 0; ks < 4; ks++) z = mfma(A2[ks * 4 + t], h1[u][ks], z);
-                    h2[u][t] = cvt4(act_scaled(z[0]), act_scaled(z[1]), act_scaled(z[2]), act_scaled(z[3]));
+                    for (int ks = 0; ks < 4; ks++) z = mfma_h(A2[ks * 4 + t], h1[u][ks], z);
+                    h2[u][t] = cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
                 f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int ks = 0; ks < 4; ks++) z = mfma(A3[ks], h2[u][ks], z);
+                for (int ks = 0; ks < 4; ks++) z = mfma_h(A3[ks], h2[u][ks], z);
                 // row 0 of the output tile: y of cell st * 16 + r16 sits in lane r16, element 0
                 const float yt = __shfl(z[0], r16, 64);
                 if (q4 == s0 + u) ymine = yt + b3;
@@ -463,7 +498,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
     WaveLds& ws = lds[wv];
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = s < n ? f2bf((X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q]) : 0;
+    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = s < n ? f2h(clamp_h((X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q])) : 0;
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
@@ -625,6 +660,25 @@ __global__ void k_mlp_pack(const float* master, bf4* pack)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // one (tile, lane) per thread
     if (t >= NPACK) return;
+    if (t >= PHF) {
+        // FP32 constants of the inference form, two per 8-byte slot:
+        // K (b2[n] + sum_k W2[k][n]) for n < 64, b3 + sum_k w3[k] at 64
+        float c[2] = {0.f, 0.f};
+        for (int e = 0; e < 2; e++) {
+            const int n = 2 * (t - PHF) + e;
+            if (n < NH) {
+                float sum = master[OB2 + n];
+                for (int k2 = 0; k2 < NH; k2++) sum += master[OW2 + k2 * NH + n];
+                c[e] = kTanhK * sum;
+            } else if (n == NH) {
+                float sum = master[OB3];
+                for (int k2 = 0; k2 < NH; k2++) sum += master[OW3 + k2];
+                c[e] = sum;
+            }
+        }
+        pack[t] = __builtin_bit_cast(bf4, (f2){c[0], c[1]});
+        return;
+    }
     const int tile = t / 64, lane = t % 64;
     const int r16 = lane & 15, q4 = lane >> 4;
     float v[4];
@@ -640,17 +694,24 @@ __global__ void k_mlp_pack(const float* master, bf4* pack)
         } else if (tile < PB3 / 64) {               // W2^T: B[k][n] = W2[n][k]
             const int q = tile - PB2T / 64, ks = q / 4, tn = q % 4;
             x = master[OW2 + (tn * 16 + r16) * NH + ks * 16 + kk];
-        } else if (tile < PB1K / 64) {              // w3 in column 0
+        } else if (tile < PH1 / 64) {               // w3 in column 0
             const int ks = tile - PB3 / 64;
             x = r16 == 0 ? master[OW3 + ks * 16 + kk] : 0.f;
-        } else if (tile < PB2K / 64) {              // K W1
-            const int n = (tile - PB1K / 64) * 16 + r16;
+        } else if (tile < PH2 / 64) {               // K W1 (FP16)
+            const int n = (tile - PH1 / 64) * 16 + r16;
             x = kk < NIN ? kTanhK * master[OW1 + kk * NH + n] : 0.f;
-        } else {                                    // K W2 [k][n]
-            const int q = tile - PB2K / 64, ks = q / 4, tn = q % 4;
-            x = kTanhK * master[OW2 + (ks * 16 + kk) * NH + tn * 16 + r16];
+        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16)
+            const int q = tile - PH2 / 64, ks = q / 4, tn = q % 4;
+            x = -2.f * kTanhK * master[OW2 + (ks * 16 + kk) * NH + tn * 16 + r16];
+        } else {                                    // -2 w3 in column 0 (FP16)
+            const int ks = tile - PH3 / 64;
+            x = r16 == 0 ? -2.f * master[OW3 + ks * 16 + kk] : 0.f;
         }
         v[e] = x;
+    }
+    if (t >= PH1) {
+        pack[t] = __builtin_bit_cast(bf4, cvt4h(v[0], v[1], v[2], v[3]));
+        return;
     }
     bf4 o;
     o.x = (short)f2bf(v[0]);
@@ -820,7 +881,6 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     a.dec = cell_decode(g);
     const long long N = (long long)g.L * g.H * g.D;
     const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
-    // one resident wave of blocks (3 per CU at this kernel's occupancy) loops over the cells
     static int ncu = 0;
     if (!ncu) {
         int dev = 0;
@@ -829,7 +889,9 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
             ncu = 256;
     }
     const int tp = mu_variant();
-    const dim3 grid((unsigned)std::min<long long>(nblk, 3LL * ncu));
+    // one resident wave of blocks (3 per CU at this kernel's occupancy) loops over the
+    // cells; a multiple of the 8 XCDs (the kernel's order assumes it)
+    const dim3 grid((unsigned)std::max<long long>(8, std::min<long long>(nblk, 3LL * ncu) / 8 * 8));
     const MlpW w = mlpw(pack, master, norm);
     if (tp == 1) hipLaunchKernelGGL(k_mlp_mu_tr<1>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
     else if (tp == 4) hipLaunchKernelGGL(k_mlp_mu_tr<4>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);

@@ -33,24 +33,34 @@ def normalise(X, norm):
 
 TANH_K = np.float32(2.8853900817779268)   # 2 log2(e): tanh(z) = 1 - 2 / (2^(K z) + 1)
 
+def f16(x):
+    """Round to FP16 (nearest even), returned as FP32."""
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
+
 
 def forward(X, params, norm, exact=False, inference=False):
     """Network output for raw features X (n x 8).  exact=True skips every
     BF16 rounding (FP32 network).  inference=True follows the inference
-    kernels' rounding: hidden layers multiply by BF16(K W) (K W formed in
-    FP32) onto K b, and tanh takes that sum divided by K; training uses
-    BF16(W) and b."""
+    kernels instead: FP16 operands, features clamped to +-60000, hidden values
+    r = 1 / (2^u + 1) of the scaled arguments u (tanh = 1 - 2 r): layer 1
+    FP16(x) FP16(K W1) + K b1, layer 2 FP16(r1) FP16(-2 K W2) +
+    K (b2 + sum_k W2[k]), output FP16(r2) FP16(-2 w3) + b3 + sum_k w3[k];
+    training uses BF16(x), BF16(W) and tanh."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
+    xn = (np.asarray(X, np.float32) - norm[:8]) * norm[8:]
+    if inference and not exact:
+        x = f16(np.clip(xn, -60000.0, 60000.0))
+        u1 = x.astype(np.float64) @ f16(TANH_K * W1) + TANH_K * b1
+        r1 = f16(1.0 / (np.exp2(u1) + 1.0))
+        c2 = TANH_K * (b2 + W2.astype(np.float32).sum(0, dtype=np.float32))
+        u2 = r1.astype(np.float64) @ f16(np.float32(-2.0) * TANH_K * W2) + c2
+        r2 = f16(1.0 / (np.exp2(u2) + 1.0))
+        y = r2.astype(np.float64) @ f16(np.float32(-2.0) * w3) + (b3 + w3.sum(dtype=np.float32))
+        return y.astype(np.float32), (x, 1.0 - 2.0 * r1, 1.0 - 2.0 * r2)
     r = (lambda a: np.asarray(a, np.float32)) if exact else bf16
-
-    def layer(h, W, b):
-        if inference and not exact:
-            return np.tanh((h.astype(np.float64) @ r(TANH_K * W) + TANH_K * b) / TANH_K)
-        return np.tanh(h.astype(np.float64) @ r(W) + b)
-
-    x = r((np.asarray(X, np.float32) - norm[:8]) * norm[8:])
-    h1 = r(layer(x, W1, b1).astype(np.float32))
-    h2 = r(layer(h1, W2, b2).astype(np.float32))
+    x = r(xn)
+    h1 = r(np.tanh(x.astype(np.float64) @ r(W1) + b1).astype(np.float32))
+    h2 = r(np.tanh(h1.astype(np.float64) @ r(W2) + b2).astype(np.float32))
     y = h2.astype(np.float64) @ r(w3) + b3
     return y.astype(np.float32), (x, h1, h2)
 
