@@ -102,8 +102,8 @@ void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long l
 void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t);
 void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D);
 int dissipation_nblocks();
-void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu,
-                            const float* dw, double* partial);
+void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu_a,
+                            const double* mu_b, double* partial_a, double* partial_b);
 void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out);   // 7 L H doubles
 void launch_diag_dw(hipStream_t s, const double* zs, const Geo& g, double dU, double dy, float fallback,
                     float* dw);

@@ -142,8 +142,7 @@ int judge_error(fsl_ctx* c, const double* mu_a, const double* mu_b, double* err,
 {
     LearnedState& L = c->lrn;
     const int nb = dissipation_nblocks();
-    launch_sgs_dissipation(c->stream, cst5(c->U, c->g.G), c->g, c->ph, mu_a, L.dw, L.part);
-    launch_sgs_dissipation(c->stream, cst5(c->U, c->g.G), c->g, c->ph, mu_b, L.dw, L.part + nb);
+    launch_sgs_dissipation(c->stream, cst5(c->U, c->g.G), c->g, c->ph, mu_a, mu_b, L.part, L.part + nb);
     int rc = check_launch(c, "dissipation");
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(L.hpart, L.part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -247,6 +247,54 @@ struct FeatArgs {
 // W, the grid-scale roughness Delta |lap u| / |S| (second differences), |S|
 // Delta / dU and y / delta_w(x).  Returns |S| = sqrt(2 S:S) (the model's
 // strain magnitude, src/turbulence.c:299-301).
+// Central-difference velocity gradient gr[r][e] = du_r/dx_e and the
+// second-difference Laplacian of u = (rho u) / rho at interior cell (i, j, k)
+// (FP32, hardware reciprocals), as in cell_features (which keeps its own copy
+// of the stencil: sharing this function cost the network kernel registers).
+__device__ __forceinline__ void velocity_gradient(const CState5& U, const Geo& g, const Phys& ph, int i, int j, int k,
+                                                  float gr[3][3], float lap[3])
+{
+#pragma clang fp contract(fast)
+    const long long c = cidx(g, i, j, k);
+    const long long st[3] = {g.si, g.sj, 1};
+    const float rc = __builtin_amdgcn_rcpf((float)U.f[0][c]);
+    float uc[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        uc[r] = (float)U.f[1 + r][c] * rc;
+        lap[r] = 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const long long p = c + st[e], m = c - st[e];
+        const float rp = __builtin_amdgcn_rcpf((float)U.f[0][p]);
+        const float rm = __builtin_amdgcn_rcpf((float)U.f[0][m]);
+        const float h = (float)ph.r2del[e];
+        const float h2 = (float)(ph.rdel[e] * ph.rdel[e]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const float up = (float)U.f[1 + r][p] * rp, um = (float)U.f[1 + r][m] * rm;
+            gr[r][e] = (up - um) * h;
+            lap[r] += (up - 2.f * uc[r] + um) * h2;
+        }
+    }
+}
+
+// S:S of the symmetric part of gr (the sum that |S| = sqrt(2 S:S) uses)
+__device__ __forceinline__ float strain_sq(const float gr[3][3])
+{
+#pragma clang fp contract(fast)
+    float s2 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            const float Sre = 0.5f * (gr[r][e] + gr[e][r]);
+            s2 += Sre * Sre;
+        }
+    return s2;
+}
+
 __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, int k, float f[NIN])
 {
     // FP32 with FMA contraction and the hardware square root (1 ulp): the
@@ -744,37 +792,42 @@ struct SumArgs {
     CState5 U;
     Geo g;
     Phys ph;
-    const double* mu;
-    const float* dw;
+    const double* mu_a;
+    const double* mu_b;
+    CellDecode dec;
 };
 
-// sum over interior cells of mu |S|^2 (SGS dissipation proxy), per block
-__global__ void __launch_bounds__(256) k_sgs_dissipation(SumArgs a, double* partial)
+// sums over interior cells of mu_a |S|^2 and mu_b |S|^2 (the judge's SGS
+// dissipation of two mu_SGS fields, one pass), per block
+__global__ void __launch_bounds__(256) k_sgs_dissipation(SumArgs a, double* partial_a, double* partial_b)
 {
-    __shared__ double red[256];
+    __shared__ double red[2][256];
     const long long N = (long long)a.g.L * a.g.H * a.g.D;
-    double s = 0.;
+    double sa = 0., sb = 0.;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
         int i, j, k;
-        decode_interior(a.g, t, i, j, k);
-        FeatArgs fa;
-        fa.U = a.U;
-        fa.g = a.g;
-        fa.ph = a.ph;
-        fa.dU = 1.;
-        fa.dw = a.dw;
-        fa.dy = 1.;
-        float f[NIN];
-        const float sm = cell_features(fa, i, j, k, f);
-        s += a.mu[cidx(a.g, i, j, k)] * (double)sm * (double)sm;
+        decode_cell(a.g, a.dec, (unsigned)t, i, j, k);
+        float gr[3][3], lap[3];
+        velocity_gradient(a.U, a.g, a.ph, i, j, k, gr, lap);
+        const float sm = __builtin_amdgcn_sqrtf(2.f * strain_sq(gr));
+        const long long c = cidx(a.g, i, j, k);
+        sa += a.mu_a[c] * (double)sm * (double)sm;
+        sb += a.mu_b[c] * (double)sm * (double)sm;
     }
-    red[threadIdx.x] = s;
+    red[0][threadIdx.x] = sa;
+    red[1][threadIdx.x] = sb;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        partial_a[blockIdx.x] = red[0][0];
+        partial_b[blockIdx.x] = red[1][0];
+    }
 }
 
 
@@ -957,16 +1010,17 @@ void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D)
 
 int dissipation_nblocks() { return 1024; }
 
-void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu,
-                            const float* dw, double* partial)
+void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu_a,
+                            const double* mu_b, double* partial_a, double* partial_b)
 {
     SumArgs a;
     a.U = U;
     a.g = g;
     a.ph = ph;
-    a.mu = mu;
-    a.dw = dw;
-    hipLaunchKernelGGL(k_sgs_dissipation, dim3(1024), dim3(256), 0, s, a, partial);
+    a.mu_a = mu_a;
+    a.mu_b = mu_b;
+    a.dec = cell_decode(g);
+    hipLaunchKernelGGL(k_sgs_dissipation, dim3(1024), dim3(256), 0, s, a, partial_a, partial_b);
 }
 
 void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out)

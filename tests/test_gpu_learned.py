@@ -234,10 +234,17 @@ def test_judge_interval_and_learned_steps():
         st = ctx.learned_status()
         S = ctx.download_state()
         err, dd, dl = ctx.learned_judge()
+        mu_dyn = ctx.download_mu()[1:-1, 1:-1, 1:-1].reshape(-1)   # the judge leaves the dynamic mu_SGS
     print(st, err, dd, dl)
     assert st["judged"] == 3 and st["fallbacks"] == 0 and st["learned_steps"] == 5
     assert np.all(np.isfinite(S))
     assert dd > 0 and dl > 0 and abs(err - abs(dl - dd) / dd) < 1e-12
+    # the judge's dissipation sum against numpy: sum mu |S|^2 with |S| = sqrt(2 S:S) of the
+    # central-difference strain (FP64 here, FP32 on the device)
+    _, smag = mlp_ref.features(S, case, u_in.max() - u_in.min(), np.ones(case.LEN + 2))
+    dd_ref = float(np.sum(mu_dyn * smag.astype(np.float64) ** 2))
+    print("judge dissipation %.9e, numpy %.9e" % (dd, dd_ref))
+    assert abs(dd - dd_ref) <= 1e-4 * dd_ref
 
 
 def test_learned_step_uses_network_mu():
