@@ -321,50 +321,33 @@ __device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, 
             lap[r] += (up - 2.f * uc[r] + um) * h2;
         }
     }
-    float S[3][3], W[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int e = 0; e < 3; e++) {
-            S[r][e] = 0.5f * (gr[r][e] + gr[e][r]);
-            W[r][e] = 0.5f * (gr[r][e] - gr[e][r]);
-        }
-    float s2 = 0.f, w2 = 0.f, s3 = 0.f, sww = 0.f, ssww = 0.f;
-    float SS[3][3], WW[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int e = 0; e < 3; e++) {
-            s2 += S[r][e] * S[r][e];
-            w2 += W[r][e] * W[r][e];
-            float a1 = 0.f, a2 = 0.f;
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                a1 += S[r][q] * S[q][e];
-                a2 += W[r][q] * W[q][e];
-            }
-            SS[r][e] = a1;
-            WW[r][e] = a2;
-        }
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int e = 0; e < 3; e++) {
-            s3 += SS[r][e] * S[e][r];
-            sww += S[r][e] * WW[e][r];
-            ssww += SS[r][e] * WW[e][r];
-        }
+    // Invariants from the six entries of the symmetric S and the vorticity
+    // om (W_ij = -eps_ijk om_k / 2, so W^2 = (om om^T - |om|^2 I) / 4):
+    //   |W|^2 = |om|^2 / 2,  tr(S W^2) = (om.S.om - |om|^2 tr S) / 4,
+    //   tr(S^2 W^2) = (om.S^2.om - |om|^2 S:S) / 4.
+    const float s00 = gr[0][0], s11 = gr[1][1], s22 = gr[2][2];
+    const float s01 = 0.5f * (gr[0][1] + gr[1][0]), s02 = 0.5f * (gr[0][2] + gr[2][0]);
+    const float s12 = 0.5f * (gr[1][2] + gr[2][1]);
+    const float om[3] = {gr[2][1] - gr[1][2], gr[0][2] - gr[2][0], gr[1][0] - gr[0][1]};
+    const float q00 = s00 * s00 + s01 * s01 + s02 * s02;   // S^2
+    const float q11 = s01 * s01 + s11 * s11 + s12 * s12;
+    const float q22 = s02 * s02 + s12 * s12 + s22 * s22;
+    const float q01 = s00 * s01 + s01 * s11 + s02 * s12;
+    const float q02 = s00 * s02 + s01 * s12 + s02 * s22;
+    const float q12 = s01 * s02 + s11 * s12 + s12 * s22;
+    const float s2 = q00 + q11 + q22;                         // S:S = tr S^2
+    const float s3 = q00 * s00 + q11 * s11 + q22 * s22 + 2.f * (q01 * s01 + q02 * s02 + q12 * s12);
+    const float oo = om[0] * om[0] + om[1] * om[1] + om[2] * om[2];
+    const float w2 = 0.5f * oo;
+    const float ws = s00 * om[0] * om[0] + s11 * om[1] * om[1] + s22 * om[2] * om[2] +
+                     2.f * (s01 * om[0] * om[1] + s02 * om[0] * om[2] + s12 * om[1] * om[2]);
+    const float wqw = q00 * om[0] * om[0] + q11 * om[1] * om[1] + q22 * om[2] * om[2] +
+                      2.f * (q01 * om[0] * om[1] + q02 * om[0] * om[2] + q12 * om[1] * om[2]);
+    const float sww = 0.25f * (ws - oo * (s00 + s11 + s22));
+    const float ssww = 0.25f * (wqw - oo * s2);
     const float tiny = 1e-30f;
     const float smag = __builtin_amdgcn_sqrtf(2.f * s2);
     const float sn = __builtin_amdgcn_sqrtf(s2);
-    const float om[3] = {gr[2][1] - gr[1][2], gr[0][2] - gr[2][0], gr[1][0] - gr[0][1]};
-    float ws = 0.f, oo = 0.f;
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        oo += om[r] * om[r];
-#pragma unroll
-        for (int e = 0; e < 3; e++) ws += om[r] * S[r][e] * om[e];
-    }
     const float yv = ((float)j - (float)(g.H / 2) - 0.5f) * (float)a.dy;
     // ratios through v_rcp_f32 (1 ulp): the network sees them rounded to BF16
     f[0] = (w2 - s2) * __builtin_amdgcn_rcpf(w2 + s2 + tiny);
