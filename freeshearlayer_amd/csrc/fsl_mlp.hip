@@ -295,28 +295,56 @@ __device__ __forceinline__ float strain_sq(const float gr[3][3])
     return s2;
 }
 
-__device__ __forceinline__ float cell_features(const FeatArgs& a, int i, int j, int k, float f[NIN])
+// State of the 7-point stencil of one cell for cell_features: ld(f, e, s)
+// is field f at the cell (s = 0) or its neighbour s = +-1 along axis e.
+struct PtrCell {   // plain pointers and 64-bit indices (any mesh)
+    CState5 U;
+    long long c, st[3];
+    __device__ __forceinline__ double ld(int f, int e, int s) const
+    {
+        return U.f[f][s == 0 ? c : (s > 0 ? c + st[e] : c - st[e])];
+    }
+};
+
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
+
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+}
+
+// One buffer resource per field over the planes a workgroup's cells and
+// their neighbours occupy (scalar registers) and a 32-bit in-window byte
+// offset per lane: no per-load 64-bit address arithmetic.
+struct BufCell {
+    __amdgpu_buffer_rsrc_t r[4];
+    unsigned oc, stb[3];
+    __device__ __forceinline__ double ld(int f, int e, int s) const
+    {
+        return bload(r[f], s == 0 ? oc : (s > 0 ? oc + stb[e] : oc - stb[e]));
+    }
+};
+
+template <class Cell>
+__device__ __forceinline__ float cell_features(const Cell& q, const FeatArgs& a, int i, int j, float f[NIN])
 {
     // FP32 with FMA contraction and the hardware square root (1 ulp): the
     // features reach the network rounded to BF16
 #pragma clang fp contract(fast)
     const Geo& g = a.g;
-    const long long c = cidx(g, i, j, k);
-    const long long st[3] = {g.si, g.sj, 1};
-    const float rc = __builtin_amdgcn_rcpf((float)a.U.f[0][c]);
+    const float rc = __builtin_amdgcn_rcpf((float)q.ld(0, 0, 0));
     float uc[3], gr[3][3], lap[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 3; r++) uc[r] = (float)a.U.f[1 + r][c] * rc;
+    for (int r = 0; r < 3; r++) uc[r] = (float)q.ld(1 + r, 0, 0) * rc;
 #pragma unroll
     for (int e = 0; e < 3; e++) {
-        const long long p = c + st[e], m = c - st[e];
-        const float rp = __builtin_amdgcn_rcpf((float)a.U.f[0][p]);
-        const float rm = __builtin_amdgcn_rcpf((float)a.U.f[0][m]);
+        const float rp = __builtin_amdgcn_rcpf((float)q.ld(0, e, 1));
+        const float rm = __builtin_amdgcn_rcpf((float)q.ld(0, e, -1));
         const float h = (float)a.ph.r2del[e];
         const float h2 = (float)(a.ph.rdel[e] * a.ph.rdel[e]);
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            const float up = (float)a.U.f[1 + r][p] * rp, um = (float)a.U.f[1 + r][m] * rm;
+            const float up = (float)q.ld(1 + r, e, 1) * rp, um = (float)q.ld(1 + r, e, -1) * rm;
             gr[r][e] = (up - um) * h;
             lap[r] += (up - 2.f * uc[r] + um) * h2;
         }
@@ -389,7 +417,8 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
     int i, j, k;
     decode_interior(a.g, cell, i, j, k);
     float f[NIN];
-    cell_features(a, i, j, k, f);
+    const PtrCell q{a.U, cidx(a.g, i, j, k), {a.g.si, a.g.sj, 1}};
+    cell_features(q, a, i, j, f);
 #pragma unroll
     for (int q = 0; q < NIN; q++) X[s * NIN + q] = f[q];
     if (t) t[s] = cd[cidx(a.g, i, j, k)] / 0.15f;
@@ -448,11 +477,26 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         if (blk >= nblk) break;   // uniform over the workgroup
         const long long cell = blk * MU_CELLS + threadIdx.x;
         const bool valid = cell < N;
+        // the block's window: from the plane before its first cell's
+        int ib, jb, kb;
+        decode_cell(a.g, a.dec, (unsigned)(blk * MU_CELLS), ib, jb, kb);
+        const long long base = (long long)(ib - 1) * a.g.si;
+        const long long wbytes = (a.g.G - base) * 8;
+        const int nrec = (int)(wbytes < 0x7fffffffLL ? wbytes : 0x7fffffffLL);
+        BufCell q;
+#pragma unroll
+        for (int fq = 0; fq < 4; fq++)
+            q.r[fq] = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.U.f[fq] + base), (short)0, nrec,
+                                                        kRsrcWord3);
+        q.stb[0] = (unsigned)a.g.si * 8u;
+        q.stb[1] = (unsigned)a.g.sj * 8u;
+        q.stb[2] = 8u;
         float f[NIN], smag = 0.f;
         int i = 1, j = 1, k = 1;
         if (valid) {
             decode_cell(a.g, a.dec, (unsigned)cell, i, j, k);
-            smag = cell_features(a, i, j, k, f);
+            q.oc = ((unsigned)(i - ib + 1) * (unsigned)a.g.si + (unsigned)j * (unsigned)a.g.sj + (unsigned)k) * 8u;
+            smag = cell_features(q, a, i, j, f);
         } else {
 #pragma unroll
             for (int q = 0; q < NIN; q++) f[q] = nm[q];
@@ -503,7 +547,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         if (valid) {
             const float Cd = fminf(fmaxf(0.15f * scale * ymine, 0.f), 0.15f);
             const long long c = cidx(a.g, i, j, k);
-            mu[c] = a.U.f[0][c] * (double)Cd * a.ph.DD * (double)smag;
+            mu[c] = q.ld(0, 0, 0) * (double)Cd * a.ph.DD * (double)smag;
             if (cd_out) cd_out[c] = Cd;
         }
         wave_lds_sync();   // X is rewritten by the next iteration
