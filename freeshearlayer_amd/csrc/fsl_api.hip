@@ -768,10 +768,10 @@ CtxView ctx_view(fsl_ctx* c)
 }
 
 void ctx_prof(fsl_ctx* c, int id, int end) { prof_mark(c, id, end); }
-int ctx_ghost(fsl_ctx* c, const double* state)
+int ctx_ghost(fsl_ctx* c, const double* state, bool z)
 {
     prof_mark(c, K_GHOST, 0);
-    launch_ghost_fill(c->stream, st5(const_cast<double*>(state), c->g.G), c->g, bnd_now(c));
+    launch_ghost_fill(c->stream, st5(const_cast<double*>(state), c->g.G), c->g, bnd_now(c), z);
     prof_mark(c, K_GHOST, 1);
     return check_launch(c, "ghost_fill");
 }

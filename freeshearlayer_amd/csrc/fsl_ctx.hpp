@@ -104,7 +104,7 @@ int ensure_sgs(fsl_ctx* c);
 SgsBufs sgs_bufs(fsl_ctx* c);
 int check_launch(fsl_ctx* c, const char* what);
 Bnd bnd_now(fsl_ctx* c);
-int ctx_ghost(fsl_ctx* c, const double* state);
+int ctx_ghost(fsl_ctx* c, const double* state, bool z = true);
 // learned closure (fsl_learned.hip)
 int learned_sgs_step(fsl_ctx* c);   // mu_SGS of the step from U (call once per step), judge, fallback
 void learned_free(fsl_ctx* c);

@@ -34,7 +34,7 @@ int learned_sgs_step(fsl_ctx* c);
 int ctx_cfl_reset(fsl_ctx* c);
 void ctx_cfl_ready(fsl_ctx* c);
 void ctx_cfl_zeroed(fsl_ctx* c, int z);
-int ctx_ghost(fsl_ctx* c, const double* state);
+int ctx_ghost(fsl_ctx* c, const double* state, bool z = true);
 void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
 
@@ -430,7 +430,10 @@ int fused_stage(fsl_ctx* ctx, int stage)
     const double* in = stage == 1 ? v.U : (stage == 2 ? v.Up : v.Uq);
     double* out = stage == 1 ? v.Up : (stage == 2 ? v.Uq : v.U);
     if (!v.Uq) return ctx_err(ctx, FSL_ERR_ARG, "fused path needs the third stage buffer (create with fused = 1)");
-    int rc = ctx_ghost(ctx, in);
+    // the mixed stage kernel reads the z-periodic images of the ghost columns
+    // of Up and Uq (the stage-1 input U keeps its full ghost layer: the learned
+    // closure's features read it)
+    int rc = ctx_ghost(ctx, in, !(v.precision == FSL_PREC_MIXED && stage > 1));
     if (rc) return rc;
     if (stage == 1 && v.sgs_model == FSL_SGS_DYNAMIC) {
         // the model reads the step state only: once per step (src/fluxes.c:53)

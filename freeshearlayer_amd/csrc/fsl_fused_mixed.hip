@@ -111,9 +111,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const long long si = g.si;
     const int pb = (int)(si * 8);                         // bytes of one plane of one field
     const unsigned sjb = (unsigned)g.sj * 8u;            // byte stride of j within a plane
-    const unsigned colb = (unsigned)jc * sjb + (unsigned)kc * 8u;
-    const unsigned oyp = (unsigned)jp * sjb + (unsigned)kc * 8u, oym = (unsigned)jm * sjb + (unsigned)kc * 8u;
-    const unsigned ozp = (unsigned)jc * sjb + (unsigned)zkp * 8u, ozm = (unsigned)jc * sjb + (unsigned)zkm * 8u;
+    // State reads take the z-periodic image of a ghost column (k = 0 -> D,
+    // k = D + 1 -> 1), the value the ghost fill copies there, so the inputs
+    // of stages 2 and 3 need no z section of the ghost fill; mu_SGS is read
+    // at its own ghost columns (colmu).
+    const int kic = kc == 0 ? D : (kc == D + 1 ? 1 : kc);
+    const int zim = zkm == 0 ? D : (zkm == D + 1 ? 1 : zkm), zip = zkp == 0 ? D : (zkp == D + 1 ? 1 : zkp);
+    const unsigned colb = (unsigned)jc * sjb + (unsigned)kic * 8u, colmu = (unsigned)jc * sjb + (unsigned)kc * 8u;
+    const unsigned oyp = (unsigned)jp * sjb + (unsigned)kic * 8u, oym = (unsigned)jm * sjb + (unsigned)kic * 8u;
+    const unsigned ozp = (unsigned)jc * sjb + (unsigned)zip * 8u, ozm = (unsigned)jc * sjb + (unsigned)zim * 8u;
     // Neighbours held by another lane of the workgroup (its own cell of the
     // plane, shared through s_w): all but the tile's edge rows / columns and
     // the periodic ghost columns (kc = 0, D + 1), which load theirs.  Lanes
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // first barrier), the next plane's y/z neighbours (next iteration).
         load5(a.in, (long long)min(i + 1, L + 1) * si, pb, colb, w2);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
-        const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colb) : 0.;
+        const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colmu) : 0.;
         // step state of the cell this iteration updates (i - 1)
         double bq[5];
         if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);

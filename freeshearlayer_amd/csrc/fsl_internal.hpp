@@ -46,7 +46,10 @@ struct Dt3 {
 };
 
 // --- unfused path (fsl_kernels.hip) ---
-void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b);
+// ghost layer of U (src/boundCondInGhostCells.c:10-96); z = false leaves the
+// z-periodic columns of the interior rows unwritten (for readers that take the
+// periodic images k = D and k = 1 instead)
+void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b, bool z = true);
 void launch_recon(hipStream_t s, CState5 U, const Faces& F, const Geo& g, int quirk);
 void launch_iface_bc(hipStream_t s, const Faces& F, const Geo& g, const Bnd& b);
 void launch_flux(hipStream_t s, int dir, CState5 U, const Faces& F, const double* mu, const Geo& g,

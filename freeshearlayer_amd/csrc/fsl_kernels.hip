@@ -373,9 +373,10 @@ __global__ void __launch_bounds__(kBlock) k_copy(const double* __restrict__ a, d
 }  // namespace
 
 // ---------------------------------------------------------------------
-void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b)
+void launch_ghost_fill(hipStream_t s, State5 U, const Geo& g, const Bnd& b, bool z)
 {
-    const long long n = 2LL * (g.H + 2) * (g.D + 2) + 2LL * g.L * (g.D + 2) + (long long)g.L * g.H;
+    // the z section (last) is left out for readers that take the periodic images
+    const long long n = 2LL * (g.H + 2) * (g.D + 2) + 2LL * g.L * (g.D + 2) + (z ? (long long)g.L * g.H : 0LL);
     hipLaunchKernelGGL(k_ghost_fill, dim3(nblk(n)), dim3(kBlock), 0, s, U, g, b);
 }
 
