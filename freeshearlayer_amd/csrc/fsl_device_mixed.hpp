@@ -279,14 +279,22 @@ __device__ __forceinline__ void face_flux(const Phys& ph, float R, const float v
 }
 
 // CFL quantity of a cell (fsl_device.hpp's cfl_cell, src/helpers.c:23-52) in
-// FP64 with one division: velocities and p / rho from 1/rho.
+// FP64: 1/rho and sqrt(K p / rho) from v_rcp_f64 and v_rsq_f64 (about 23
+// correct bits) refined by one Newton step each, about 1e-14 relative (the
+// quantity only sets dt = CFL / max); zero and negative K p / rho give 0 and
+// NaN as sqrt does.
 __device__ __forceinline__ double cfl_cell(int mode, const double q[5], const Phys& ph, double dtX)
 {
-    const double Ri = 1.0 / q[0];
+    const double R = q[0];
+    double Ri = __builtin_amdgcn_rcp(R);
+    Ri = __builtin_fma(Ri, __builtin_fma(-R, Ri, 1.0), Ri);
     if (mode == 0) return q[1] * Ri * dtX;
     const double u = q[1] * Ri, vv = q[2] * Ri, w = q[3] * Ri;
-    const double p = (q[4] - 0.5 * q[0] * (u * u + vv * vv + w * w)) * kK_1;
-    const double cs = sqrt(kK * p * Ri);
+    const double p = (q[4] - 0.5 * R * (u * u + vv * vv + w * w)) * kK_1;
+    const double t = kK * p * Ri;
+    double y = __builtin_amdgcn_rsq(t);
+    y = y * __builtin_fma(-0.5 * t * y, y, 1.5);
+    const double cs = t > 0.0 ? t * y : (t == 0.0 ? 0.0 : __builtin_nan(""));
     return (fabs(u) + cs) * ph.rdel[0] + (fabs(vv) + cs) * ph.rdel[1] + (fabs(w) + cs) * ph.rdel[2];
 }
 
