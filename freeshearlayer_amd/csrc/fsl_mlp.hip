@@ -45,8 +45,14 @@ constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
 // FP16 operands of the inference kernels (K = 2 log2 e, the tanh argument
 // scale; the hidden layers carry r = 1 / (2^u + 1), tanh = 1 - 2 r):
 constexpr int PH1 = PB3 + 4 * 64;       // K W1
-constexpr int PH2 = PH1 + 4 * 64;       // -2 K W2
-constexpr int PH3 = PH2 + 16 * 64;      // -2 w3 in column 0
+constexpr int PH2 = PH1 + 4 * 64;       // -2 K W2, 16x16x32 operands (below)
+constexpr int PH3 = PH2 + 16 * 64;      // -2 w3 in column 0, 16x16x32 operands
+// Layers 2 and 3 use v_mfma_f32_16x16x32_f16: the operand of K slab s (input
+// units 32 s .. 32 s + 31), output tile t and lane l is the pair of 16x16x16
+// operands of K tiles 2 s and 2 s + 1 (units 32 s + 4 (l / 16) + e and
+// 32 s + 16 + 4 (l / 16) + e), 16 bytes: the A and B elements an MFMA pairs
+// share lane group and element, so this unit order is the product's K order,
+// and the previous layer's two output tiles concatenate into the B operand.
 constexpr int PHF = PH3 + 4 * 64;       // FP32: K (b2 + sum_k W2[k][n]) (64), b3 + sum_k w3[k] (1)
 constexpr int NPACK = PHF + 64;
 
@@ -82,6 +88,14 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 
 __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4 mfma_h32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ h8 cat8(h4 a, h4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+// 16x16x32 operand (K slab s, tile t) of lane l in a packed region (two 8-byte units)
+__device__ __forceinline__ h8 pack8(const bf4* region, int s, int t, int ntiles, int lane)
+{
+    return *reinterpret_cast<const h8*>(region + 2 * ((s * ntiles + t) * 64 + lane));
+}
 __device__ __forceinline__ h4 as_h4(bf4 v) { return __builtin_bit_cast(h4, v); }
 
 // tanh(z + b) = 1 - 2 / (2^(K (z + b)) + 1), K = 2 log2(e), with kb = K b:
@@ -169,11 +183,16 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
             const float c2 = cf[tn * 16 + r16];
             acc[tn] = INFER ? f4{c2, c2, c2, c2} : f4{0.f, 0.f, 0.f, 0.f};
         }
-        for (int ks = 0; ks < 4; ks++) {
-            const bf4 a = lds_row4(&ws.H1[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
-            for (int tn = 0; tn < 4; tn++) {
-                if (INFER) acc[tn] = mfma_h(as_h4(a), as_h4(w.pack[PH2 + (ks * 4 + tn) * 64 + lane]), acc[tn]);
-                else acc[tn] = mfma(a, w.pack[PB2 + (ks * 4 + tn) * 64 + lane], acc[tn]);
+        if (INFER) {
+            for (int sl = 0; sl < 2; sl++) {
+                const h8 a = cat8(as_h4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
+                                  as_h4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
+                for (int tn = 0; tn < 4; tn++) acc[tn] = mfma_h32(a, pack8(w.pack + PH2, sl, tn, 4, lane), acc[tn]);
+            }
+        } else {
+            for (int ks = 0; ks < 4; ks++) {
+                const bf4 a = lds_row4(&ws.H1[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
+                for (int tn = 0; tn < 4; tn++) acc[tn] = mfma(a, w.pack[PB2 + (ks * 4 + tn) * 64 + lane], acc[tn]);
             }
         }
         for (int tn = 0; tn < 4; tn++) {
@@ -188,10 +207,17 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
     // layer 3: H2 (64 x 64) w3 (64 x 16, column 0)
     for (int tm = 0; tm < 4; tm++) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ks++) {
-            const bf4 a = lds_row4(&ws.H2[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
-            if (INFER) acc = mfma_h(as_h4(a), as_h4(w.pack[PH3 + ks * 64 + lane]), acc);
-            else acc = mfma(a, w.pack[PB3 + ks * 64 + lane], acc);
+        if (INFER) {
+            for (int sl = 0; sl < 2; sl++) {
+                const h8 a = cat8(as_h4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
+                                  as_h4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
+                acc = mfma_h32(a, pack8(w.pack + PH3, sl, 0, 1, lane), acc);
+            }
+        } else {
+            for (int ks = 0; ks < 4; ks++) {
+                const bf4 a = lds_row4(&ws.H2[(tm * 16 + r16) * NH + ks * 16 + 4 * q4]);
+                acc = mfma(a, w.pack[PB3 + ks * 64 + lane], acc);
+            }
         }
         if (r16 == 0) {
             const float b3 = INFER ? cf[64] : w.master[OB3];
@@ -441,13 +467,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
     __shared__ unsigned short sX[4][64 * NIN];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q4 = lane >> 4;
-    h4 A1[4], A2[16], A3[4];
+    h4 A1[4];
+    h8 A2[2][4], A3[2];
     float kb1[4][4], kb2[4][4];
     const float* cf = reinterpret_cast<const float*>(w.pack + PHF);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         A1[t] = as_h4(w.pack[PH1 + t * 64 + lane]);
-        A3[t] = as_h4(w.pack[PH3 + t * 64 + lane]);
+        A2[0][t] = pack8(w.pack + PH2, 0, t, 4, lane);
+        A2[1][t] = pack8(w.pack + PH2, 1, t, 4, lane);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             kb1[t][e] = kTanhK * w.master[OB1 + t * 16 + 4 * q4 + e];
@@ -455,7 +483,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         }
     }
 #pragma unroll
-    for (int t = 0; t < 16; t++) A2[t] = as_h4(w.pack[PH2 + t * 64 + lane]);
+    for (int sl = 0; sl < 2; sl++) A3[sl] = pack8(w.pack + PH3, sl, 0, 1, lane);
     const float b3 = cf[64];
     float nm[NIN], ns[NIN];
 #pragma unroll
@@ -530,15 +558,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
                     f4 z = {kb2[t][0], kb2[t][1], kb2[t][2], kb2[t][3]};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ks++) z = mfma_h(A2[ks * 4 + t], h1[u][ks], z);
+                    z = mfma_h32(A2[0][t], cat8(h1[u][0], h1[u][1]), z);
+                    z = mfma_h32(A2[1][t], cat8(h1[u][2], h1[u][3]), z);
                     h2[u][t] = cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
                 f4 z = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int ks = 0; ks < 4; ks++) z = mfma_h(A3[ks], h2[u][ks], z);
+                z = mfma_h32(A3[0], cat8(h2[u][0], h2[u][1]), z);
+                z = mfma_h32(A3[1], cat8(h2[u][2], h2[u][3]), z);
                 // row 0 of the output tile: y of cell st * 16 + r16 sits in lane r16, element 0
                 const float yt = __shfl(z[0], r16, 64);
                 if (q4 == s0 + u) ymine = yt + b3;
@@ -775,12 +803,16 @@ __global__ void k_mlp_pack(const float* master, bf4* pack)
         } else if (tile < PH2 / 64) {               // K W1 (FP16)
             const int n = (tile - PH1 / 64) * 16 + r16;
             x = kk < NIN ? kTanhK * master[OW1 + kk * NH + n] : 0.f;
-        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16)
-            const int q = tile - PH2 / 64, ks = q / 4, tn = q % 4;
-            x = -2.f * kTanhK * master[OW2 + (ks * 16 + kk) * NH + tn * 16 + r16];
-        } else {                                    // -2 w3 in column 0 (FP16)
-            const int ks = tile - PH3 / 64;
-            x = r16 == 0 ? -2.f * master[OW3 + ks * 16 + kk] : 0.f;
+        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16), 16x16x32 order
+            const int u = t - PH2, pos = u >> 1, half = u & 1;
+            const int sl = pos / 256, tn = (pos / 64) % 4, ln = pos % 64;
+            const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e, nn = tn * 16 + (ln & 15);
+            x = -2.f * kTanhK * master[OW2 + (ks * 16 + kq) * NH + nn];
+        } else {                                    // -2 w3 in column 0 (FP16), 16x16x32 order
+            const int u = t - PH3, pos = u >> 1, half = u & 1;
+            const int sl = pos / 64, ln = pos % 64;
+            const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e;
+            x = (ln & 15) == 0 ? -2.f * master[OW3 + ks * 16 + kq] : 0.f;
         }
         v[e] = x;
     }
