@@ -20,7 +20,7 @@ Prints one JSON line.  `value` is the headline path's throughput;
 ratio.  `roofline` prices the dominant kernel's algorithmic bytes per
 launch against its average launch duration, measured with HIP events on the
 context's stream over the timed steps; `mlp_roofline` the learned-closure
-kernel's MFMA FLOPs against the BF16 dense peak.  `traffic` (HBM bytes per
+kernel's algorithmic FLOPs against the FP16 dense peak (its products are FP16).  `traffic` (HBM bytes per
 launch) and the VALU / MFMA busy fractions come from rocprofv3 counter runs
 of the same command (profiles/).  `cpu_baseline` times the
 FP64 CPU oracle (the reference algorithm, one host core) on a bounded
@@ -46,7 +46,7 @@ import freeshearlayer_amd as fsl  # noqa: E402
 from freeshearlayer_amd import shear_case  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-BF16_PEAK_TFLOPS = 2500.0
+F16_PEAK_TFLOPS = 2500.0   # MI355X dense FP16 / BF16 MFMA
 MODEL = os.path.join(ROOT, "freeshearlayer_amd", "data", "sgs_mlp.bin")
 METRIC = "Mcell-updates/s per RK stage on 1 MI355X, FP32+learned SGS vs FP64 dyn-Smag"
 UNIT = "Mcell-updates/s per RK stage"
@@ -228,8 +228,8 @@ def main(argv=None):
         ms, cnt = leg["prof"]["mlp_mu"]
         tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
         tab = profile_table()
-        mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
+        mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": F16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tf / F16_PEAK_TFLOPS, 5),
                         "flop_per_cell": MLP_FLOP_PER_CELL, "mfma_flop_per_cell": MLP_MFMA_FLOP_PER_CELL,
                         "avg_launch_us": round(ms / cnt * 1e3, 2),
                         "mfma_busy_pct": tab.get("mfma_busy@mlp_mu@%s" % wl),
