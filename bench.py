@@ -211,7 +211,7 @@ def main(argv=None):
         ctx.close()
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; the domain is not sharded across GPUs "
+        print("bench.py: the per-step update runs on one GPU; the domain is not sharded across GPUs "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         return 2
     L, H, D = shear_case.parse_mesh(a.mesh)
