@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libfsl.so")
 LAYER2_PATH = os.path.join(LIB_DIR, "libfsl_layer2.so")
 
 FSL_OK = 0
+# error codes of include/fsl.h
+FSL_ERR_ARG, FSL_ERR_HIP, FSL_ERR_NOMEM, FSL_ERR_NONFINITE, FSL_ERR_IO, FSL_ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 FSL_SGS_CONSTANT, FSL_SGS_DYNAMIC, FSL_SGS_LEARNED = 0, 1, 2
 FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
 FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1

@@ -1,0 +1,27 @@
+"""Size limits of fsl_create, checked before any device call (runs without a
+GPU): the mixed-precision stage kernel addresses one (j, k) plane of a field
+with 32-bit byte offsets, the learned closure decodes interior cells from a
+31-bit index; beyond them fsl_create returns FSL_ERR_UNSUPPORTED."""
+import pytest
+
+import freeshearlayer_amd as fsl
+
+
+def test_mixed_path_rejects_a_plane_beyond_32_bit_offsets():
+    case = fsl.case_reference(4, 16384, 16384)     # (H + 2)(D + 2) * 8 bytes > 2^31 - 1
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
+        fsl.Context(case, fsl.config(precision=fsl.FSL_PREC_MIXED, fused=1))
+
+
+def test_learned_closure_rejects_more_than_2_31_cells():
+    case = fsl.case_reference(2048, 1024, 1024)    # 2^31 interior cells
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
+        fsl.Context(case, fsl.config(sgs_model=fsl.FSL_SGS_LEARNED, fused=1))
+
+
+def test_error_codes_match_the_header():
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "fsl.h")).read()
+    codes = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define (FSL_ERR_\w+)\s+(-?\d+)", hdr))
+    assert codes and all(getattr(fsl, k) == v for k, v in codes.items()), codes
