@@ -28,6 +28,7 @@ struct MixedArgs {
     Bnd b;
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+    unsigned fb;                   // bytes of one field (G doubles) in the PK form
     unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (FP64; or null)
     unsigned long long* cfl_zero;  // stage 2: CFL cell to clear for the following stage 3 (or null)
     int cfl_mode;
@@ -46,25 +47,39 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const double* field
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(field + plane), (short)0, bytes, kRsrcWord3);
 }
 
-__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff)
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff, unsigned soff = 0)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, (int)soff, 0));
 }
 
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned boff, double v)
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned boff, unsigned soff, double v)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)boff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)boff, (int)soff, 0);
 }
 
-__device__ __forceinline__ void load5(const CState5& s, long long plane, int pbytes, unsigned boff, double q[5])
+// Field f of a state buffer's plane.  PK: one resource per plane (base =
+// field 0 + plane, range = the four further fields + one plane) and the
+// field's byte offset f * fb in the scalar offset, which needs 4 fb + one
+// plane below 2^32 bytes; otherwise one resource per field and plane.  PK
+// holds 4 scalar registers per buffer instead of 20.
+template <bool PK, class S5>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const S5& s, int f, long long plane, int pbytes,
+                                                           unsigned fb)
+{
+    return PK ? plane_rsrc(s.f[0], plane, (int)(4u * fb + (unsigned)pbytes)) : plane_rsrc(s.f[f], plane, pbytes);
+}
+
+template <bool PK>
+__device__ __forceinline__ void load5(const CState5& s, long long plane, int pbytes, unsigned fb, unsigned boff,
+                                      double q[5])
 {
 #pragma unroll
-    for (int f = 0; f < 5; f++) q[f] = bload(plane_rsrc(s.f[f], plane, pbytes), boff);
+    for (int f = 0; f < 5; f++) q[f] = bload(field_rsrc<PK>(s, f, plane, pbytes, fb), boff, PK ? f * fb : 0u);
 }
 
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 
-template <int STAGE, int BJ, int BK, int MINW>
+template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 {
     constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
@@ -111,6 +126,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const long long si = g.si;
     const int pb = (int)(si * 8);                         // bytes of one plane of one field
     const unsigned sjb = (unsigned)g.sj * 8u;            // byte stride of j within a plane
+    const unsigned fb = a.fb;
     // State reads take the z-periodic image of a ghost column (k = 0 -> D,
     // k = D + 1 -> 1), the value the ghost fill copies there, so the inputs
     // of stages 2 and 3 need no z section of the ghost fill; mu_SGS is read
@@ -152,8 +168,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     for (int f = 0; f < 5; f++) { Fx_prev[f] = 0.f; ly[f] = 0.f; lz[f] = 0.f; }
     {
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
-        load5(a.in, (long long)ia * si, pb, colb, w1);
-        load5(a.in, (long long)ib * si, pb, colb, w2);
+        load5<PK>(a.in, (long long)ia * si, pb, fb, colb, w1);
+        load5<PK>(a.in, (long long)ib * si, pb, fb, colb, w2);
 #pragma unroll
         for (int f = 0; f < 5; f++) { pye[f] = 0.; pze[f] = 0.; pz2[f] = 0.; }
 #pragma unroll
@@ -174,12 +190,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // order): next plane's cell and this cell's mu_SGS (end of the
         // reconstruction), the step state of the updated cell (after the
         // first barrier), the next plane's y/z neighbours (next iteration).
-        load5(a.in, (long long)min(i + 1, L + 1) * si, pb, colb, w2);
+        load5<PK>(a.in, (long long)min(i + 1, L + 1) * si, pb, fb, colb, w2);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
         const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colmu) : 0.;
         // step state of the cell this iteration updates (i - 1)
         double bq[5];
-        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si, pb, colb, bq);
+        if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, colb, bq);
         // y and z neighbours of this plane's cell: from s_w (written by their
         // lanes in the previous iteration) or, on the edge lanes, loaded one
         // iteration ahead
@@ -194,9 +210,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 nzm[f] = zmin ? s_w[sl][f][czm] : pze[f];
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
-            if (yload) load5(a.in, pn, pb, oye, pye);
-            if (zload) load5(a.in, pn, pb, oze, pze);
-            if (z2load) load5(a.in, pn, pb, ozp, pz2);
+            if (yload) load5<PK>(a.in, pn, pb, fb, oye, pye);
+            if (zload) load5<PK>(a.in, pn, pb, fb, oze, pze);
+            if (z2load) load5<PK>(a.in, pn, pb, fb, ozp, pz2);
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
@@ -301,7 +317,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     if (STAGE == 1) o = s;
                     else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
                     else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
-                    bstore(plane_rsrc(a.out.f[f], po, pb), colb, o);
+                    bstore(field_rsrc<PK>(a.out, f, po, pb, fb), colb, PK ? f * fb : 0u, o);
                     ov[f] = o;
                 }
                 if (STAGE == 3 && a.cfl_out) {
@@ -394,7 +410,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     if (STAGE == 3 && a.cfl_out) cfl_fold(cfl_m, a.cfl_out);
 }
 
-template <int BJ, int BK, int MINW>
+template <int BJ, int BK, int MINW, bool PK>
 void launch_mixed(hipStream_t s, int stage, MixedArgs a, int per_cu)
 {
     const int TJ = BJ - 2, TK = BK - 2;
@@ -408,9 +424,9 @@ void launch_mixed(hipStream_t s, int stage, MixedArgs a, int per_cu)
     a.nblocks = nseg * cols;
     a.xcd_remap = 1;
     const dim3 grid((a.nblocks + 7) / 8 * 8), block(BK, BJ);
-    if (stage == 1) hipLaunchKernelGGL((k_fused_mixed<1, BJ, BK, MINW>), grid, block, 0, s, a);
-    else if (stage == 2) hipLaunchKernelGGL((k_fused_mixed<2, BJ, BK, MINW>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_fused_mixed<3, BJ, BK, MINW>), grid, block, 0, s, a);
+    if (stage == 1) hipLaunchKernelGGL((k_fused_mixed<1, BJ, BK, MINW, PK>), grid, block, 0, s, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_fused_mixed<2, BJ, BK, MINW, PK>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused_mixed<3, BJ, BK, MINW, PK>), grid, block, 0, s, a);
 }
 
 }  // namespace
@@ -436,7 +452,17 @@ void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* o
     // 16 x 16 workgroups at two waves per SIMD (two workgroups per CU, whose
     // barriers overlap); 32 x 16 and 16 x 32 tiles (one workgroup per CU)
     // and three waves per SIMD (register spills) measured slower
-    launch_mixed<16, 16, 2>(v.stream, stage, a, 2);
+    // one buffer resource per state plane where the five fields fit 32-bit
+    // offsets, (4 G + one plane) * 8 < 2^32 bytes: up to about 1.3e8 cells;
+    // FSL_MIXED_FIELD_RSRC=1 selects the per-field form at any size
+    const unsigned long long span = (4ull * (unsigned long long)v.g.G + (unsigned long long)v.g.si) * 8ull;
+    const char* e = getenv("FSL_MIXED_FIELD_RSRC");
+    if (span <= 0xffffffffull && !(e && atoi(e) == 1)) {
+        a.fb = (unsigned)(v.g.G * 8);
+        launch_mixed<16, 16, 2, true>(v.stream, stage, a, 2);
+    } else {
+        launch_mixed<16, 16, 2, false>(v.stream, stage, a, 2);
+    }
 }
 
 }  // namespace fsl

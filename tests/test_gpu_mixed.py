@@ -99,3 +99,22 @@ def test_fp32_dynamic_model_vs_fp64():
     gh = np.ones_like(m32, dtype=bool)
     gh[I] = False
     assert np.all(m32[gh] == 0)
+
+
+def test_stage_kernel_buffer_forms_bit_identical(monkeypatch):
+    """The stage kernel addresses a state plane's five fields through one
+    buffer resource and scalar field offsets where they fit 32-bit offsets
+    (every mesh of this suite), else through one resource per field
+    (FSL_MIXED_FIELD_RSRC=1 forces that form): the same loads and stores, so
+    the states after several steps agree bit for bit, on a mesh whose sizes
+    are no multiple of the tile."""
+    case = fsl.case_reference(40, 30, 22)
+    U, u_in, mu = fsl.ic_shear_layer(case)
+    case.mu_L = mu
+    monkeypatch.delenv("FSL_MIXED_FIELD_RSRC", raising=False)
+    a, dta, _ = run(case, U, u_in, fsl.FSL_PREC_MIXED, fsl.FSL_SGS_DYNAMIC, 6)
+    monkeypatch.setenv("FSL_MIXED_FIELD_RSRC", "1")
+    b, dtb, _ = run(case, U, u_in, fsl.FSL_PREC_MIXED, fsl.FSL_SGS_DYNAMIC, 6)
+    assert np.all(np.isfinite(a))
+    assert np.array_equal(a, b)
+    assert np.array_equal(dta, dtb)
