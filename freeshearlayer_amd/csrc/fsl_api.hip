@@ -234,6 +234,11 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         return FSL_ERR_ARG;
     }
     const int L = cs->LEN, H = cs->HIG, D = cs->DEP;
+    if (L < 1 || H < 1 || D < 1) {
+        fprintf(stderr, "fsl_create: mesh %d x %d x %d has an empty direction\n", L, H, D);
+        delete c;
+        return FSL_ERR_ARG;
+    }
     c->g.L = L; c->g.H = H; c->g.D = D;
     c->g.sj = D + 2;
     c->g.si = (long long)(H + 2) * (D + 2);

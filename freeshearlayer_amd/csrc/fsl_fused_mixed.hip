@@ -82,7 +82,7 @@ __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 {
-    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
+    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK, WK = BK + 2, NW = (BJ + 2) * WK;
     __shared__ float s_vr[3][3][NT];    // 1/rho velocities, planes i-1, i, i+1
     __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
     __shared__ float s_ym[4][NT];       // - face states along y: density, momenta
@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
     __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
-    __shared__ double s_w[2][5][NT];    // state of planes i, i+1 (slot p & 1): y/z neighbours
+    __shared__ double s_w[2][5][NW];    // state of planes i, i+1 (slot p & 1) with a ring of outer
+                                        // neighbours: y/z neighbours
 
     const Geo& g = a.g;
     const int L = g.L, H = g.H, D = g.D;
@@ -120,43 +121,38 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const bool yface = tx >= 1 && tx <= TK && ty <= TJ && k <= D && j <= H;
     const bool zface = ty >= 1 && ty <= TJ && tx <= TK && j <= H && k <= D;
     const int jm = max(jc - 1, 0), jp = min(jc + 1, H + 1);
-    int zkm = kc - 1, zkp = kc + 1;
-    if (kc == 0) { zkm = D - 1; zkp = D + 1; }
-    else if (kc == D + 1) { zkm = 0; zkp = 2; }
     const long long si = g.si;
     const int pb = (int)(si * 8);                         // bytes of one plane of one field
     const unsigned sjb = (unsigned)g.sj * 8u;            // byte stride of j within a plane
     const unsigned fb = a.fb;
-    // State reads take the z-periodic image of a ghost column (k = 0 -> D,
-    // k = D + 1 -> 1), the value the ghost fill copies there, so the inputs
-    // of stages 2 and 3 need no z section of the ghost fill; mu_SGS is read
-    // at its own ghost columns (colmu).
-    const int kic = kc == 0 ? D : (kc == D + 1 ? 1 : kc);
-    const int zim = zkm == 0 ? D : (zkm == D + 1 ? 1 : zkm), zip = zkp == 0 ? D : (zkp == D + 1 ? 1 : zkp);
-    const unsigned colb = (unsigned)jc * sjb + (unsigned)kic * 8u, colmu = (unsigned)jc * sjb + (unsigned)kc * 8u;
-    const unsigned oyp = (unsigned)jp * sjb + (unsigned)kic * 8u, oym = (unsigned)jm * sjb + (unsigned)kic * 8u;
-    const unsigned ozp = (unsigned)jc * sjb + (unsigned)zip * 8u, ozm = (unsigned)jc * sjb + (unsigned)zim * 8u;
-    // Neighbours held by another lane of the workgroup (its own cell of the
-    // plane, shared through s_w): all but the tile's edge rows / columns and
-    // the periodic ghost columns (kc = 0, D + 1), which load theirs.  Lanes
-    // clamped beyond the domain (j > H + 1, k > D + 1) may take a clamped
-    // neighbour's state: their face states are not used.
-    const bool zin = kc >= 1 && kc <= D;
-    const bool ypin = ty < BJ - 1, ymin = ty > 0, zpin = tx < BK - 1 && zin, zmin = tx > 0 && zin;
-    const int cyp = ypin ? c + BK : c, cym = ymin ? c - BK : c, czp = zpin ? c + 1 : c, czm = zmin ? c - 1 : c;
-    // Edge lanes load one neighbour per direction (a lane lacks at most one y
-    // side and, unless it is a periodic ghost column, one z side), so each
-    // direction needs one load set with a per-lane offset; ghost columns load
-    // their second z neighbour into a third set.
-    const bool yload = !ypin || !ymin, zload = !zpin || !zmin, z2load = !zpin && !zmin;
-    const unsigned oye = ymin ? oyp : oym, oze = zmin ? ozp : ozm;
+    // State reads take the z-periodic image of a column, k -> (k - 1) mod D + 1
+    // (ghost columns k = 0 -> D, k = D + 1 -> 1: the values the ghost fill
+    // copies there, so the inputs of stages 2 and 3 need no z section of the
+    // ghost fill; lanes beyond the domain, k > D + 1, hold the image of their
+    // column, so that the lane of k = D + 1 finds its z + 1 neighbour, the
+    // image of k = D + 2, in the next lane).  mu_SGS is read at its own ghost
+    // columns (colmu).
+    const int kim = ((k - 1) % D + D) % D + 1;
+    const unsigned colb = (unsigned)jc * sjb + (unsigned)kim * 8u, colmu = (unsigned)jc * sjb + (unsigned)kc * 8u;
+    // y/z neighbours of a lane's cell come from s_w: the cells of the other
+    // lanes and a ring of outer neighbours that the edge rows and columns of
+    // the tile load (one load set per direction: a lane lacks at most one y
+    // and one z side).  In y the rows are clamped (j - 1 >= 0, j + 1 <= H + 1),
+    // in z the columns periodic; every neighbour a used face state needs is
+    // the one the reference's ghost layer holds there.
+    const int cw = (ty + 1) * WK + tx + 1;
+    const bool yedge = ty == 0 || ty == BJ - 1, zedge = tx == 0 || tx == BK - 1;
+    const int cwy = ty == 0 ? cw - WK : cw + WK, cwz = tx == 0 ? cw - 1 : cw + 1;
+    const int kie = (((tx == 0 ? k - 1 : k + 1) - 1) % D + D) % D + 1;
+    const unsigned oye = (unsigned)(ty == 0 ? jm : jp) * sjb + (unsigned)kim * 8u;
+    const unsigned oze = (unsigned)jc * sjb + (unsigned)kie * 8u;
     const float r4[3] = {(float)a.ph.r4del[0], (float)a.ph.r4del[1], (float)a.ph.r4del[2]};
     const float rd[3] = {(float)a.ph.rdel[0], (float)a.ph.rdel[1], (float)a.ph.rdel[2]};
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
-    double pye[5], pze[5], pz2[5];           // edge lanes: y/z neighbours of the next plane's cell
+    double pye[5], pze[5];                   // edge lanes: outer y/z neighbours of the next plane
     float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
     float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
     mx::FaceF xp_prev;
@@ -170,8 +166,6 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
         load5<PK>(a.in, (long long)ia * si, pb, fb, colb, w1);
         load5<PK>(a.in, (long long)ib * si, pb, fb, colb, w2);
-#pragma unroll
-        for (int f = 0; f < 5; f++) { pye[f] = 0.; pze[f] = 0.; pz2[f] = 0.; }
 #pragma unroll
         for (int f = 0; f < 5; f++) dxp[f] = (float)(w2[f] - w1[f]);
         ri1 = mx::rcp((float)w2[0]);
@@ -189,30 +183,29 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // Loads in the order of first use (the wait counter retires in issue
         // order): next plane's cell and this cell's mu_SGS (end of the
         // reconstruction), the step state of the updated cell (after the
-        // first barrier), the next plane's y/z neighbours (next iteration).
+        // first barrier), the next plane's outer y/z neighbours (before the
+        // second barrier).
         load5<PK>(a.in, (long long)min(i + 1, L + 1) * si, pb, fb, colb, w2);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
         const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colmu) : 0.;
         // step state of the cell this iteration updates (i - 1)
         double bq[5];
         if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, colb, bq);
-        // y and z neighbours of this plane's cell: from s_w (written by their
-        // lanes in the previous iteration) or, on the edge lanes, loaded one
-        // iteration ahead
+        // y and z neighbours of this plane's cell, written to s_w in the
+        // previous iteration
         double nyp[5], nym[5], nzp[5], nzm[5];
         {
             const int sl = i & 1;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
-                nyp[f] = ypin ? s_w[sl][f][cyp] : pye[f];
-                nym[f] = ymin ? s_w[sl][f][cym] : pye[f];
-                nzp[f] = zpin ? s_w[sl][f][czp] : (zmin ? pze[f] : pz2[f]);
-                nzm[f] = zmin ? s_w[sl][f][czm] : pze[f];
+                nyp[f] = s_w[sl][f][cw + WK];
+                nym[f] = s_w[sl][f][cw - WK];
+                nzp[f] = s_w[sl][f][cw + 1];
+                nzm[f] = s_w[sl][f][cw - 1];
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
-            if (yload) load5<PK>(a.in, pn, pb, fb, oye, pye);
-            if (zload) load5<PK>(a.in, pn, pb, fb, oze, pze);
-            if (z2load) load5<PK>(a.in, pn, pb, fb, ozp, pz2);
+            if (yedge) load5<PK>(a.in, pn, pb, fb, oye, pye);
+            if (zedge) load5<PK>(a.in, pn, pb, fb, oze, pze);
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
@@ -266,7 +259,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             ri1 = ri;
             s_mu[vc][c] = mu_c;
 #pragma unroll
-            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][c] = w2[f];
+            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cw] = w2[f];
         }
         lds_barrier();
 
@@ -397,6 +390,16 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
+        }
+        // outer neighbours of plane i + 1 (the slot's last reads were the
+        // previous plane's, before its barriers)
+        if (yedge) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cwy] = pye[f];
+        }
+        if (zedge) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cwz] = pze[f];
         }
         lds_barrier();
         if (inseg && interior) {

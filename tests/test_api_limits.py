@@ -1,7 +1,8 @@
 """Size limits of fsl_create, checked before any device call (runs without a
 GPU): the mixed-precision stage kernel addresses one (j, k) plane of a field
 with 32-bit byte offsets, the learned closure decodes interior cells from a
-31-bit index; beyond them fsl_create returns FSL_ERR_UNSUPPORTED."""
+31-bit index; beyond them fsl_create returns FSL_ERR_UNSUPPORTED.  A mesh
+with an empty direction is FSL_ERR_ARG."""
 import pytest
 
 import freeshearlayer_amd as fsl
@@ -17,6 +18,13 @@ def test_learned_closure_rejects_more_than_2_31_cells():
     case = fsl.case_reference(2048, 1024, 1024)    # 2^31 interior cells
     with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
         fsl.Context(case, fsl.config(sgs_model=fsl.FSL_SGS_LEARNED, fused=1))
+
+
+@pytest.mark.parametrize("mesh", [(0, 8, 8), (8, 0, 8), (8, 8, 0), (8, -1, 8)])
+def test_create_rejects_an_empty_direction(mesh):
+    case = fsl.case_reference(*mesh)
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_ARG):
+        fsl.Context(case, fsl.config(fused=1))
 
 
 def test_error_codes_match_the_header():
