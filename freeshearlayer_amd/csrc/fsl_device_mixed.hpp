@@ -31,8 +31,12 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 // compare/select chain).
 __device__ __forceinline__ float minmod(float x, float y) { return __builtin_amdgcn_fmed3f(x, 0.f, 4.f * y); }
 
-// pressure from FP64 total energy and an FP32 kinetic energy
-__device__ __forceinline__ float pressure(double E, float ke) { return (float)((E - (double)ke) * kK_1); }
+// pressure from FP64 total energy and an FP32 rho |u|^2, halved and
+// subtracted in one FP64 fma
+__device__ __forceinline__ float pressure2(double E, float rho_uu)
+{
+    return (float)(__builtin_fma((double)rho_uu, -0.5, E) * kK_1);
+}
 
 // face state: FP32 density and momenta, FP64 total energy (pressure recovery)
 struct FaceF {
@@ -58,12 +62,19 @@ __device__ __forceinline__ void outflow_face(double p_out, FaceF& q)
 
 struct PrimF {
     float R, vel[3], P, C, aa, ll, ee, gg, hh, nn, S11;
-    float vhh[3], vee[3];
+    float vhh[3], vee[3], vc[3];
 };
+
+// Limiter weights (1 + kappa)/4 and (1 - kappa)/4 at kappa = 1/3: the FP32
+// values satisfy C1 = 2 C2 exactly, so a face increment C1 a + C2 b is
+// C2 (2a + b), and C2 is carried by the right-eigenvector coefficients.
+constexpr float kC2f = (float)kC2;
+static_assert((float)kC1 == 2.f * (float)kC2, "limiter weights");
 
 // Per-cell quantities of the characteristic transform from the state and its
 // 1/rho and velocity (computed once per cell by the caller); C and 1/C come
-// from one reciprocal square root of K p / rho.
+// from one reciprocal square root of K p / rho.  The right-eigenvector
+// coefficients (ll, ee, gg, hh, nn, vhh, vee, vc) include the factor C2.
 __device__ __forceinline__ void cell_prims(const double q[5], float Ri, const float v[3], PrimF& c)
 {
 #pragma clang fp contract(fast)
@@ -73,12 +84,12 @@ __device__ __forceinline__ void cell_prims(const double q[5], float Ri, const fl
     c.vel[1] = v[1];
     c.vel[2] = v[2];
     const float ff = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
-    c.P = pressure(q[4], 0.5f * r * ff);
+    c.P = pressure2(q[4], r * ff);
     const float C2 = 1.4f * c.P * c.R;
     const float rC = __builtin_amdgcn_rsqf(C2);
     c.C = C2 * rC;
     c.aa = 0.2f * ff;
-    c.ll = 0.5f * rC;
+    c.ll = (0.5f * kC2f) * rC;
     c.ee = c.ll * rC;
     c.gg = -ff * c.ee;
     c.hh = -(c.ee + c.ee);
@@ -87,6 +98,7 @@ __device__ __forceinline__ void cell_prims(const double q[5], float Ri, const fl
     for (int r2 = 0; r2 < 3; r2++) {
         c.vhh[r2] = c.vel[r2] * c.hh;
         c.vee[r2] = c.vel[r2] * c.ee;
+        c.vc[r2] = c.vel[r2] * kC2f;
     }
     c.nn = -0.5f * c.gg;
 }
@@ -132,30 +144,30 @@ __device__ __forceinline__ void recon_dir_d(const PrimF& c, const double q[5], c
         Wh[r] = minmod(W[r], M[r]);
     }
     const float S24 = c.vee[N - 1] + c.ll, S25 = c.vee[N - 1] - c.ll;
-    const float S5 = c.nn + 1.25f;
+    const float S5 = c.nn + 1.25f * kC2f;
     const float S54 = S5 + mm, S55 = S5 - mm;
     const float ee_t2 = quirk ? c.vee[N - 1] : c.vee[T2 - 1];
-    const float C1f = (float)kC1, C2f = (float)kC2;
     float d[5];
+    // kv = (C1 Mh + C2 Wh) / C2 for the + face, (C2 Mh + C1 Wh) / C2 for the - face
 #pragma unroll
-    for (int r = 0; r < 5; r++) kv[r] = Mh[r] * C1f + Wh[r] * C2f;
+    for (int r = 0; r < 5; r++) kv[r] = 2.f * Mh[r] + Wh[r];
     kk = kv[3] + kv[4];
     d[0] = c.hh * kv[0] + c.ee * kk;
     d[N] = c.vhh[N - 1] * kv[0] + S24 * kv[3] + S25 * kv[4];
-    d[T1] = c.vhh[T1 - 1] * kv[0] + kv[1] + c.vee[T1 - 1] * kk;
-    d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
-    d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
+    d[T1] = c.vhh[T1 - 1] * kv[0] + kC2f * kv[1] + c.vee[T1 - 1] * kk;
+    d[T2] = c.vhh[T2 - 1] * kv[0] + kC2f * kv[2] + ee_t2 * kk;
+    d[4] = c.gg * kv[0] + c.vc[T1 - 1] * kv[1] + c.vc[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) plus.f[r] = qf[r] + d[r];
     plus.E = q[4] + (double)d[4];
 #pragma unroll
-    for (int r = 0; r < 5; r++) kv[r] = Mh[r] * C2f + Wh[r] * C1f;
+    for (int r = 0; r < 5; r++) kv[r] = Mh[r] + 2.f * Wh[r];
     kk = kv[3] + kv[4];
     d[0] = c.hh * kv[0] + c.ee * kk;
     d[N] = c.vhh[N - 1] * kv[0] + S24 * kv[3] + S25 * kv[4];
-    d[T1] = c.vhh[T1 - 1] * kv[0] + kv[1] + c.vee[T1 - 1] * kk;
-    d[T2] = c.vhh[T2 - 1] * kv[0] + kv[2] + ee_t2 * kk;
-    d[4] = c.gg * kv[0] + c.vel[T1 - 1] * kv[1] + c.vel[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
+    d[T1] = c.vhh[T1 - 1] * kv[0] + kC2f * kv[1] + c.vee[T1 - 1] * kk;
+    d[T2] = c.vhh[T2 - 1] * kv[0] + kC2f * kv[2] + ee_t2 * kk;
+    d[4] = c.gg * kv[0] + c.vc[T1 - 1] * kv[1] + c.vc[T2 - 1] * kv[2] + S54 * kv[3] + S55 * kv[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) minus.f[r] = qf[r] - d[r];
     minus.E = q[4] - (double)d[4];
@@ -186,7 +198,8 @@ __device__ __forceinline__ SideF face_side(const FaceF& q)
     SideF s;
     const float R = q.f[0], Ri = rcp(R);
     s.v[0] = q.f[1] * Ri; s.v[1] = q.f[2] * Ri; s.v[2] = q.f[3] * Ri;
-    s.P = pressure(q.E, 0.5f * R * (s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2]));
+    // rho |u|^2 as momentum . velocity
+    s.P = pressure2(q.E, q.f[1] * s.v[0] + q.f[2] * s.v[1] + q.f[3] * s.v[2]);
     const float C2 = 1.4f * s.P * Ri;
     s.t = __builtin_amdgcn_rsqf(C2);
     s.C = C2 * s.t;
@@ -235,9 +248,10 @@ __device__ __forceinline__ void face_flux(const Phys& ph, float R, const float v
                                           const float g[3][3], float mu_dyn, float F[5])
 {
 #pragma clang fp contract(fast)
-    const float uu = 0.5f * (vf[0] + vb[0]);
-    const float vv = 0.5f * (vf[1] + vb[1]);
-    const float ww = 0.5f * (vf[2] + vb[2]);
+    // twice the face velocity (the stress work takes the factor 1/2 once)
+    const float uu = vf[0] + vb[0];
+    const float vv = vf[1] + vb[1];
+    const float ww = vf[2] + vb[2];
     float mu_T;
     if (ph.dynamic) {
         mu_T = mu_dyn;
@@ -266,16 +280,17 @@ __device__ __forceinline__ void face_flux(const Phys& ph, float R, const float v
         s1 = mu_E * (g[2][1] + g[1][2]);
         s2 = t3 * mu_E * (g[2][2] + g[2][2] - g[0][0] - g[1][1]);
     }
-    const float lambda_E = (float)ph.lambda_L + mu_T * (float)ph.cp_Pr_T;
-    const float q = -lambda_E * (Tf - Tb) * (float)ph.rdel[DIR];
+    // heat flux -(lambda_L + mu_T cp/Pr_T) (Tf - Tb) / delta, the constants scaled by -1/delta
+    const float lq = mu_T * (float)(-ph.cp_Pr_T * ph.rdel[DIR]) + (float)(-ph.lambda_L * ph.rdel[DIR]);
+    const float q = lq * (Tf - Tb);
     const float RU = R * vel[DIR];
     F[0] = RU;
     F[1] = RU * vel[0] - s0;
     F[2] = RU * vel[1] - s1;
     F[3] = RU * vel[2] - s2;
     F[1 + DIR] += P;
-    F[4] = vel[DIR] * (3.5f * P + 0.5f * R * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) - uu * s0 -
-           vv * s1 - ww * s2 + q;
+    F[4] = vel[DIR] * (3.5f * P + 0.5f * R * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) -
+           0.5f * (uu * s0 + vv * s1 + ww * s2) + q;
 }
 
 // CFL quantity of a cell (fsl_device.hpp's cfl_cell, src/helpers.c:23-52) in
