@@ -118,3 +118,23 @@ def test_stage_kernel_buffer_forms_bit_identical(monkeypatch):
     assert np.all(np.isfinite(a))
     assert np.array_equal(a, b)
     assert np.array_equal(dta, dtb)
+
+
+@pytest.mark.parametrize("mesh", [(24, 14, 14), (20, 28, 28), (18, 15, 15), (20, 12, 8), (40, 10, 1), (5, 10, 3)])
+def test_mixed_one_step_at_tile_edges(mesh):
+    """Meshes where the stage kernel's 16 x 16 tiles end exactly on the ghost
+    row / column (HIG, DEP = 14, 28: the ghost lane is the tile's edge lane,
+    which loads its outer neighbour), where lanes beyond the periodic end
+    hold wrapped images (DEP = 8, 15), and the thinnest directions (DEP = 1,
+    3): one mixed step against the FP64 path, the bound of the 64x48x32
+    test."""
+    L, H, D = mesh
+    case = fsl.case_reference(L, H, D, BL_HIG=min(5, H - H // 2))
+    U, u_in, mu = fsl.ic_shear_layer(case)
+    case.mu_L = mu
+    a, _, _ = run(case, U, u_in, fsl.FSL_PREC_MIXED, fsl.FSL_SGS_DYNAMIC, 1)
+    b, _, _ = run(case, U, u_in, fsl.FSL_PREC_FP64, fsl.FSL_SGS_DYNAMIC, 1)
+    e = rel_err(a, b)
+    print("mesh %s, one step: %s" % (mesh, ["%.2e" % x for x in e]))
+    assert np.all(np.isfinite(a))
+    assert max(e) < 2e-6, e
