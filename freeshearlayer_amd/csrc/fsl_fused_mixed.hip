@@ -35,10 +35,10 @@ struct MixedArgs {
 };
 
 // Plane-relative buffer access.  The x index of a plane is the same for the
-// whole workgroup, so each field's plane is one buffer resource in scalar
-// registers (base = field + plane, range = one plane) and a lane supplies only
-// its 32-bit in-plane byte offset: no per-lane 64-bit address arithmetic, and
-// the hardware range check confines every access to that plane.
+// whole workgroup, so a state buffer's plane is a buffer resource in scalar
+// registers (field_rsrc below) and a lane supplies only its 32-bit in-plane
+// byte offset: no per-lane 64-bit address arithmetic, and the hardware range
+// check confines every access to the plane's fields.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
 
@@ -209,10 +209,11 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
-        // minus states) was last read by the previous plane's face loops,
-        // which every thread finished before the previous plane's last
-        // barrier; s_fy/s_fz, read after that barrier, are rewritten only
-        // after the next one.
+        // minus states) was last read by the previous plane's face loops, and
+        // the s_w slot of plane i + 1 (= slot of i - 1) at the start of the
+        // previous plane, all before the previous plane's last barrier;
+        // s_fy/s_fz, read after that barrier, are rewritten only after the
+        // next one.
         const int vc = i & 1, vb = vc ^ 1;
         mx::PrimF P;
         mx::cell_prims(w1, ri1, v1, P);
