@@ -22,9 +22,12 @@ launch against its average launch duration, measured with HIP events on the
 context's stream over the timed steps; `mlp_roofline` the learned-closure
 kernel's algorithmic FLOPs against the FP16 dense peak (its products are FP16).  `traffic` (HBM bytes per
 launch) and the VALU / MFMA busy fractions come from rocprofv3 counter runs
-of the same command (profiles/).  `cpu_baseline` times the
-FP64 CPU oracle (the reference algorithm, one host core) on a bounded
-sample -- a reported baseline, not the target.
+of the same command (profiles/).  `cpu_baseline` times the reference's
+own serial path (src/, compiled from its sources into
+oracle/_ref/libfslref_dyn.so with real = double and zero-initialised
+arrays) on one host core and a bounded sample, or, where that library was
+not built, the FP64 CPU oracle (the same algorithm restated in C; also
+reported as `cpu_port`) -- a reported baseline, not the target.
 
     python bench.py [--gpus 1] [--steps 300] [--warmup 120] [--mesh 384x256x256]
                     [--sgs learned|dynamic|constant] [--precision mixed|fp64]
@@ -181,6 +184,35 @@ def cpu_baseline(steps=2, mesh=(192, 128, 96)):
             "cpu": model, "host_logical_cores": ncpu}
 
 
+def cpu_reference(steps=4, mesh=(192, 128, 96)):
+    """The compiled reference (oracle/_ref/libfslref_dyn.so: src/ with real =
+    double, FP64 dynamic Smagorinsky, rows zero-initialised) stepping the same
+    shear-layer state on one core: `steps` of its `layer2` steps (checkCoNum,
+    block draw, 3 stages of ghost fill, reconstruction, interface states,
+    fluxes with the dynamic model, evolution).  Its own inflow and time-step
+    rule apply; the per-cell cost does not depend on them.  None when the
+    library was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import RefLib, ref_available
+    if not ref_available("dyn"):
+        return None
+    L, H, D = mesh
+    c, U, _, _ = shear_case.initial(L, H, D, mach=0.4)
+    ref = RefLib("dyn")
+    ref.initialize(L, H, D, mu_L="%.17g" % c.mu_L, BL_HIG=c.BL_HIG)
+    ref.set_state(U)
+    ref.step()                       # first touch of every array
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ref.step()
+    t = time.perf_counter() - t0
+    model, ncpu = host_cpu()
+    return {"value": round(L * H * D * 3 * steps / t / 1e6, 4), "unit": UNIT, "cores": 1, "kind": "reference",
+            "sample": "%d steps of the reference's serial src/ path (compiled from its sources, FP64, dynamic "
+                      "SGS) on the same shear layer at %dx%dx%d (%.1f s)" % (steps, L, H, D, t),
+            "cpu": model, "host_logical_cores": ncpu}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,10 +277,14 @@ def main(argv=None):
         compare = {"value": round(ref["value"], 2), "ms_per_step": round(ref["t"] / a.steps * 1e3, 4),
                    "workload": workload_name("fp64", "dynamic", a.path, L, H, D, a.mach), "roofline": rl}
         ref["ctx"].close()
-    cpu = None
+    cpu = port = None
     if not a.no_cpu:
+        print("[bench] CPU reference sample", file=sys.stderr, flush=True)
+        cpu = cpu_reference()
         print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline()
+        port = cpu_baseline()
+        if cpu is None:
+            cpu = port
     line = {
         "metric": METRIC,
         "value": round(leg["value"], 2),
@@ -267,6 +303,7 @@ def main(argv=None):
         "speedup_vs_fp64_dynamic": round(leg["value"] / compare["value"], 3) if compare else None,
         "learned_status": leg["status"],
         "cpu_baseline": cpu,
+        "cpu_port": port,
         "kernels": kernels,
         "final_dt": final_dt,
     }

@@ -27,3 +27,18 @@ def test_help_lists_the_workload_options():
     assert r.returncode == 0
     for opt in ("--steps", "--warmup", "--mesh", "--sgs", "--precision", "--mach", "--no-compare"):
         assert opt in r.stdout
+
+
+def test_cpu_baselines_on_a_small_sample():
+    """Both CPU legs of the bench run on the host (no GPU): the compiled
+    reference (where oracle/_ref was built) and the oracle."""
+    import pytest
+    sys.path.insert(0, ROOT)
+    import bench
+    from oracle_lib import ref_available
+    port = bench.cpu_baseline(steps=1, mesh=(16, 12, 8))
+    assert port["kind"] == "port" and port["cores"] == 1 and port["value"] > 0
+    if not ref_available("dyn"):
+        pytest.skip("compiled reference not built here")
+    ref = bench.cpu_reference(steps=1, mesh=(16, 12, 8))
+    assert ref["kind"] == "reference" and ref["cores"] == 1 and ref["value"] > 0
