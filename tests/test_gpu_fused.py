@@ -98,3 +98,33 @@ def test_hundred_steps_shear_layer_divergence(dynamic):
             assert max(div) == 0.0, "step %d: %s" % (n, div)
     orc.close()
     assert worst == 0.0
+
+
+def test_fused_matches_oracle_at_192x128x96():
+    """The GPU configuration's mesh: the shear layer on 192 x 128 x 96 cells
+    with dynamic Smagorinsky, two steps of the fused FP64 path against the
+    CPU oracle, bit for bit (state and dt; the oracle needs a few seconds per
+    step here)."""
+    from oracle_lib import Oracle
+    from freeshearlayer_amd import shear_case
+    L, H, D = 192, 128, 96
+    case, U, u_in, shear = shear_case.initial(L, H, D)
+    orc = Oracle(L, H, D, dynamic=1)
+    orc.set("mu_L", case.mu_L)
+    orc.set("BL_HIG", case.BL_HIG)
+    orc.u_in()[:] = u_in
+    orc.derive()
+    orc.set_state(U)
+    orc.set("X", 0.317391239817)
+    cfg = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC, fused=1)
+    with fsl.Context(case, cfg) as ctx:
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        ctx.set_lcg(0.317391239817, 0)
+        for n in range(1, 3):
+            ctx.step()
+            orc.step()
+            assert ctx.dt == orc.get("dt"), "dt at step %d" % n
+            assert_bitwise(orc.state()[:, 1:-1, 1:-1, 1:-1], ctx.download_state()[:, 1:-1, 1:-1, 1:-1],
+                           "interior U after step %d" % n)
+    orc.close()
