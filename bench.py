@@ -2,7 +2,8 @@
 
 Default workload: the FP32 + learned-closure path (mixed precision: FP32
 stencil arithmetic, FP64 state / pressure recovery / RK / CFL; the per-cell
-BF16-MFMA network for Cs^2, judged against the dynamic model every 50
+network for Cs^2 on the matrix cores (FP16 operands, FP32 accumulation;
+trained in BF16), judged against the dynamic model every 50
 steps with a dynamic fallback above 10 % error) on the synthetic free shear
 layer at 384x256x256 cells, Mc 0.4 -- tanh mean profile U(y) = Um + dU/2
 tanh(2y/delta_w0), U2/U1 = 0.5, Re 500 on delta_w0 = 4 dy, seeded 2 % dU
@@ -292,7 +293,7 @@ def main(argv=None):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(leg["t"] / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64+bf16" if a.sgs == "learned" else "f32+f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64+f16" if a.sgs == "learned" else "f32+f64",
         "data": "synthetic shear layer (tanh profile, Mc %.1f, U2/U1 0.5, Re 500, 2%% seeded perturbations, "
                 "acoustic CFL %.2f)" % (a.mach, a.cfl),
         "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
