@@ -8,7 +8,7 @@
  * fluxes.o, turbulence.o, evolution.o, helpers.o).  It adds three calls:
  * fsl_layer2_attach() after Initialize(), fsl_layer2_set_block() after each
  * disturbance-block draw, fsl_layer2_sync_host() before Probes()/Output()/
- * Finalize().  The state then lives on the GPU; see INTEGRATION.md.
+ * Finalize().  The state then lives on the GPU.
  *
  * The functions keep the reference's void signatures; a failure is recorded
  * and returned by fsl_layer2_status() (the reference has no error returns).

@@ -9,8 +9,8 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 
 def _ensure_built():
-    """Build the native libraries once if they are missing (the driver
-    normally runs __graft_entry__.build() first)."""
+    """Build the native libraries once if they are missing (normally
+    __graft_entry__.build() has built them)."""
     import subprocess
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)

@@ -127,7 +127,7 @@ def test_g6_two_stage_fallthrough():
                        "Up after step %d" % n)
 
 
-def test_survey_block_golden():
+def test_block_draw_golden():
     """First block of the reference run log for X0 and DEP = 32: counter 41,
     k_min 3, k_max 11, Ud 19.824909, Vd -1.837488, Wd -30.123856."""
     orc = Oracle(4, 4, 32)
