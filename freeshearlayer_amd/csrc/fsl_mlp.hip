@@ -454,8 +454,8 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
 // Z^T = W^T X^T with units as MFMA rows (M) and cells as columns (N).  The
 // output tile D[unit][cell] then sits in the lanes exactly as the next
 // layer's B operand B[unit][cell] (lane = cell + 16 (unit / 4), element =
-// unit % 4), so activations go from the accumulators through bias, tanh and
-// BF16 rounding straight into the next product.  The weight tiles are the
+// unit % 4), so activations go from the accumulators through the activation
+// (act_r) and FP16 rounding straight into the next product.  The weight tiles are the
 // A operands (same lane layout as the packed B tiles) and stay in registers
 // for the block's whole grid-stride loop; only the 8 features of each cell
 // pass through LDS (one 8-byte read per lane and 16-cell tile).
