@@ -9,10 +9,15 @@ layer at 384x256x256 cells, Mc 0.4 -- tanh mean profile U(y) = Um + dU/2
 tanh(2y/delta_w0), U2/U1 = 0.5, Re 500 on delta_w0 = 4 dy, seeded 2 % dU
 perturbations where |y| < delta_w0, acoustic CFL 0.5, 3-stage TVD RK --
 timed beside the plain FP64 dynamic-Smagorinsky path on the same grid and
-initial state.  Inputs are resident in HBM before each timed region; the
-120 untimed warm-up steps of each leg carry the run past the judges of the
-synthetic initial state (laminar profile plus white noise), which the
-learned closure leaves to its dynamic fallback.
+initial state.  Inputs are resident in HBM before each timed region.
+
+Before its --warmup steps the learned leg runs an untimed spin-up that does
+not depend on --warmup or --steps: it steps until a judge has passed and the
+closure runs on the network (at most --spinup-cap steps).  Its learned-closure
+counters are read before and after the timed steps; the timed window's own
+counts are reported (`learned_window`), and if any timed step ran on the
+dynamic fallback the line carries value null and the exit status is 1: the
+headline is only ever a measurement of the network path.
 
 Metric: Mcell-updates/s per RK stage = cells x stages x steps / seconds.
 
@@ -30,7 +35,7 @@ arrays) on one host core and a bounded sample, or, where that library was
 not built, the FP64 CPU oracle (the same algorithm restated in C; also
 reported as `cpu_port`) -- a reported baseline, not the target.
 
-    python bench.py [--gpus 1] [--steps 300] [--warmup 120] [--mesh 384x256x256]
+    python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--mesh 384x256x256]
                     [--sgs learned|dynamic|constant] [--precision mixed|fp64]
                     [--mach 0.4] [--path fused|unfused] [--no-compare] [--no-cpu]
 """
@@ -106,19 +111,44 @@ def workload_name(precision, sgs, path, L, H, D, mach):
         "FP32+FP64" if precision == "mixed" else "FP64", sgs, path, L, H, D, mach)
 
 
-def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=True):
+def spin_up(ctx, cap):
+    """Untimed steps until the learned closure has passed a judge and runs on
+    the network (the synthetic initial state -- laminar profile plus white
+    noise -- may fail the first judge, which leaves the next steps to the
+    dynamic fallback).  Returns the number of steps taken; raises when the
+    closure is still in fallback after `cap` steps."""
+    n = 0
+    while True:
+        st = ctx.learned_status()
+        if st["judged"] >= 1 and not st["in_fallback"] and st["learned_steps"] > 0:
+            return n
+        if n >= cap:
+            raise RuntimeError("learned closure still on the dynamic fallback after %d spin-up steps "
+                               "(status %s)" % (n, st))
+        ctx.step()
+        n += 1
+
+
+WINDOW_KEYS = ("judged", "fallbacks", "learned_steps", "dynamic_steps")
+
+
+def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=True, spinup_cap=400,
+             presteps=0):
     ctx, case, U0, u_in, _ = shear_case.make(L, H, D, sgs=sgs, precision=precision, fused=(path == "fused"),
                                              cfl=cfl, mach=mach)
+    spun = 0
     if sgs == "learned":
         p, n = fsl.mlp_file_read(MODEL)
         ctx.mlp_set_params(p, n)
         ctx.learned_configure(50, 0.10)
-    for _ in range(warmup):
+        spun = spin_up(ctx, spinup_cap)
+    for _ in range(presteps + warmup):
         ctx.step()
     ctx.synchronize()
     ctx.profile_reset()
     ctx.profile(profile)
     ctx.synchronize()
+    before = ctx.learned_status() if sgs == "learned" else None
     t0 = time.perf_counter()
     ctx.run(steps)
     ctx.synchronize()
@@ -126,13 +156,63 @@ def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=Tr
     ctx.profile(False)
     prof = ctx.profile_read()
     sums = ctx.conservation_sums()
-    status = ctx.learned_status() if sgs == "learned" else None
+    status = window = None
+    if sgs == "learned":
+        status = ctx.learned_status()
+        window = {k: status[k] - before[k] for k in WINDOW_KEYS}
+        window["steps"] = steps
+        window["in_fallback_at_end"] = status["in_fallback"]
     out = {"ctx": ctx, "case": case, "U0": U0, "u_in": u_in, "t": t, "prof": prof, "sums": sums,
-           "status": status, "value": L * H * D * 3 * steps / t / 1e6}
+           "status": status, "window": window, "spinup_steps": spun,
+           "value": L * H * D * 3 * steps / t / 1e6}
     return out
 
 
-def roofline_of(prof, L, H, D, sgs, workload):
+# Sources whose text determines each profiled kernel's machine code: a
+# counter figure recorded for other sources is stale and is not reported.
+KERNEL_SOURCES = {
+    ("fused_stage", "mixed"): ["fsl_fused_mixed.hip", "fsl_device_mixed.hpp", "fsl_device.hpp"],
+    ("fused_stage", "fp64"): ["fsl_fused.hip", "fsl_device.hpp"],
+    ("mlp_mu", "mixed"): ["fsl_mlp.hip"],
+    ("mlp_mu", "fp64"): ["fsl_mlp.hip"],
+}
+
+
+def sources_sha16(kernel, precision):
+    import hashlib
+    files = KERNEL_SOURCES.get((kernel, precision))
+    if not files:
+        return None
+    h = hashlib.sha256()
+    for f in files:
+        with open(os.path.join(ROOT, "freeshearlayer_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_table():
+    """Per-kernel figures measured with rocprofv3 counters (profiles/)."""
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    return json.load(open(tpath)).get("entries", {}) if os.path.exists(tpath) else {}
+
+
+def counter_figures(kernel, precision, workload):
+    """(traffic, valu_busy, mfma_busy, provenance) of `kernel` on `workload`
+    from profiles/hbm_traffic.json; the figures are None unless they were
+    measured on kernels built from the current sources."""
+    e = profile_table().get("%s@%s" % (kernel, workload))
+    if e is None:
+        return None, None, None, {"source": None, "note": "no counter run recorded for this workload"}
+    prov = {"source": "profiles/hbm_traffic.json (rocprofv3 --pmc, %s)" % e.get("measured"),
+            "sources_sha16": e.get("sources_sha16")}
+    cur = sources_sha16(kernel, precision)
+    if cur is None or cur != e.get("sources_sha16"):
+        prov["note"] = "stale: the kernel sources changed since the counter run (now %s)" % cur
+        return None, None, None, prov
+    return e.get("traffic"), e.get("valu_busy_pct"), e.get("mfma_busy_pct"), prov
+
+
+def roofline_of(prof, L, H, D, sgs, precision, workload):
     if not prof:
         return None, {}
     kernels = {k: {"ms_total": round(v[0], 3), "launches": int(v[1]), "avg_us": round(v[0] / v[1] * 1e3, 2)}
@@ -142,19 +222,11 @@ def roofline_of(prof, L, H, D, sgs, workload):
     if b is None:
         return None, kernels
     avg_s = ms / cnt / 1e3
-    prof_tab = profile_table()
-    traffic = prof_tab.get("%s@%s" % (name, workload))
+    traffic, valu, _, prov = counter_figures(name, precision, workload)
     ach = b / avg_s / 1e9
     return ({"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
-             "alg_bytes_per_launch": int(b),
-             "valu_busy_pct": prof_tab.get("valu_busy@%s@%s" % (name, workload))}, kernels)
-
-
-def profile_table():
-    """Per-kernel figures measured with rocprofv3 counters (profiles/)."""
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    return json.load(open(tpath)) if os.path.exists(tpath) else {}
+             "alg_bytes_per_launch": int(b), "valu_busy_pct": valu, "counters": prov}, kernels)
 
 
 def cpu_baseline(steps=2, mesh=(192, 128, 96)):
@@ -218,9 +290,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    # untimed: past the judges of the synthetic initial state (laminar profile
-    # + white noise), which the learned closure leaves to the dynamic fallback
-    ap.add_argument("--warmup", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--spinup-cap", type=int, default=400,
+                    help="most untimed steps the learned leg may take to leave the dynamic fallback")
     ap.add_argument("--mesh", default="384x256x256")
     ap.add_argument("--sgs", default="learned", choices=["learned", "dynamic", "constant"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
@@ -250,30 +322,39 @@ def main(argv=None):
     L, H, D = shear_case.parse_mesh(a.mesh)
     wl = workload_name(a.precision, a.sgs, a.path, L, H, D, a.mach)
     print("[bench] %s: %d warmup + %d timed steps" % (wl, a.warmup, a.steps), file=sys.stderr, flush=True)
-    leg = time_leg(L, H, D, a.sgs, a.precision, a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile)
+    leg = time_leg(L, H, D, a.sgs, a.precision, a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile,
+                   spinup_cap=a.spinup_cap)
     ctx = leg["ctx"]
+    failures = []
     if not np.all(np.isfinite(leg["sums"])):
         print("bench.py: non-finite state after the timed steps", file=sys.stderr)
         return 1
-    roofline, kernels = roofline_of(leg["prof"], L, H, D, a.sgs, wl)
+    roofline, kernels = roofline_of(leg["prof"], L, H, D, a.sgs, a.precision, wl)
     mlp_roofline = None
     if "mlp_mu" in leg["prof"]:
         ms, cnt = leg["prof"]["mlp_mu"]
         tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
-        tab = profile_table()
+        _, valu, mfma, prov = counter_figures("mlp_mu", a.precision, wl)
         mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": F16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tf / F16_PEAK_TFLOPS, 5),
                         "flop_per_cell": MLP_FLOP_PER_CELL, "mfma_flop_per_cell": MLP_MFMA_FLOP_PER_CELL,
-                        "avg_launch_us": round(ms / cnt * 1e3, 2),
-                        "mfma_busy_pct": tab.get("mfma_busy@mlp_mu@%s" % wl),
-                        "valu_busy_pct": tab.get("valu_busy@mlp_mu@%s" % wl)}
+                        "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": int(cnt),
+                        "mfma_busy_pct": mfma, "valu_busy_pct": valu, "counters": prov}
+    window = leg["window"]
+    if a.sgs == "learned":
+        if window["dynamic_steps"] or window["fallbacks"] or window["learned_steps"] != a.steps:
+            failures.append("timed steps ran on the dynamic fallback: %s" % window)
+        if mlp_roofline is None and not a.no_profile:
+            failures.append("no learned-closure kernel launch was timed")
     final_dt = ctx.dt
     ctx.close()
     compare = None
     if not a.no_compare and not (a.sgs == "dynamic" and a.precision == "fp64"):
         print("[bench] comparison leg: FP64 dynamic-SGS, same grid", file=sys.stderr, flush=True)
-        ref = time_leg(L, H, D, "dynamic", "fp64", a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile)
-        rl, _ = roofline_of(ref["prof"], L, H, D, "dynamic",
+        # as many untimed steps as the headline leg took, so both time the same flow
+        ref = time_leg(L, H, D, "dynamic", "fp64", a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile,
+                       presteps=leg["spinup_steps"])
+        rl, _ = roofline_of(ref["prof"], L, H, D, "dynamic", "fp64",
                             workload_name("fp64", "dynamic", a.path, L, H, D, a.mach))
         compare = {"value": round(ref["value"], 2), "ms_per_step": round(ref["t"] / a.steps * 1e3, 4),
                    "workload": workload_name("fp64", "dynamic", a.path, L, H, D, a.mach), "roofline": rl}
@@ -285,10 +366,13 @@ def main(argv=None):
         print("[bench] CPU oracle sample", file=sys.stderr, flush=True)
         port = cpu_baseline()
         if cpu is None:
+            failures.append("the compiled reference (oracle/_ref/libfslref_dyn.so) is missing: "
+                            "cpu_baseline would be the restatement")
             cpu = port
+    value = None if failures else round(leg["value"], 2)
     line = {
         "metric": METRIC,
-        "value": round(leg["value"], 2),
+        "value": value,
         "unit": UNIT,
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(leg["t"] / a.steps * 1e3, 4),
@@ -301,15 +385,22 @@ def main(argv=None):
         "roofline": roofline,
         "mlp_roofline": mlp_roofline,
         "fp64_dynamic": compare,
-        "speedup_vs_fp64_dynamic": round(leg["value"] / compare["value"], 3) if compare else None,
+        "speedup_vs_fp64_dynamic": (round(leg["value"] / compare["value"], 3)
+                                    if compare and value is not None else None),
+        "spinup_steps": leg["spinup_steps"],
+        "learned_window": window,
         "learned_status": leg["status"],
         "cpu_baseline": cpu,
         "cpu_port": port,
         "kernels": kernels,
         "final_dt": final_dt,
     }
+    if failures:
+        line["failures"] = failures
+        for f in failures:
+            print("bench.py: %s" % f, file=sys.stderr)
     print(json.dumps(line))
-    return 0
+    return 1 if failures else 0
 
 
 if __name__ == "__main__":

@@ -183,6 +183,14 @@ def load():
         "fsl_learned_get_status": (i_, [vp, C.POINTER(FslLearnedStatus)]),
         "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
         "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
+        "fsl_output_fields": (i_, [vp, i_, i_, _dp]),
+        "fsl_write_plt": (i_, [vp, cc]),
+        "fsl_plt_write_fields": (i_, [cc, C.POINTER(FslCase), _dp]),
+        "fsl_probe_values": (i_, [vp, _dp]),
+        "fsl_write_probes": (i_, [vp, cc, i_]),
+        "fsl_probes_append": (i_, [cc, u_, u_, _dp, i_]),
+        "fsl_favre_average": (i_, [cc, u_, u_, u_, i_, _dp, C.POINTER(u_)]),
+        "fsl_favre_write_profiles": (i_, [cc, u_, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -516,12 +524,67 @@ class Context:
         self._chk(self.lib.fsl_learned_judge(self.h, C.byref(e), C.byref(dd), C.byref(dl)), "learned_judge")
         return e.value, dd.value, dl.value
 
+    def output_fields(self, k0=1, nk=None):
+        """Output()'s nine computed columns (rho, u, v, w, p, T, |omega|, Q,
+        mu_T/mu_L) of planes k0 .. k0+nk-1: array (nk, HIG, LEN, 9)."""
+        case = self.case
+        if nk is None:
+            nk = case.DEP - k0 + 1
+        out = np.zeros((nk, case.HIG, case.LEN, 9))
+        self._chk(self.lib.fsl_output_fields(self.h, k0, nk, _ptr(out)), "output_fields")
+        return out
+
+    def write_plt(self, path=None):
+        self._chk(self.lib.fsl_write_plt(self.h, path.encode() if path else None), "write_plt")
+
+    def probe_values(self):
+        out = np.zeros((6, self.case.HIG))
+        self._chk(self.lib.fsl_probe_values(self.h, _ptr(out)), "probe_values")
+        return out
+
+    def write_probes(self, path=None, real_bytes=8):
+        self._chk(self.lib.fsl_write_probes(self.h, path.encode() if path else None, real_bytes), "write_probes")
+
     def mfma_probe(self, A, B):
         A = _f32(A, (16, 16))
         B = _f32(B, (16, 16))
         D = np.zeros((16, 16), np.float32)
         self._chk(self.lib.fsl_mfma_probe(self.h, _fptr(A), _fptr(B), _fptr(D)), "mfma_probe")
         return D
+
+
+def plt_write_fields(path, case, fields):
+    """The Tecplot frame of Output() from host fields (LEN HIG DEP cells x 9)."""
+    fields = np.ascontiguousarray(fields, dtype=np.float64)
+    assert fields.size == 9 * case.LEN * case.HIG * case.DEP
+    rc = load().fsl_plt_write_fields(path.encode(), C.byref(case), _ptr(fields))
+    if rc:
+        raise FslError("fsl_plt_write_fields (%d)" % rc)
+
+
+def probes_append(path, step, vals, real_bytes=8):
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    H = vals.size // 6
+    rc = load().fsl_probes_append(path.encode(), step, H, _ptr(vals), real_bytes)
+    if rc:
+        raise FslError("fsl_probes_append (%d)" % rc)
+
+
+def favre_average(path, HIG, step_start, step_end, real_bytes=4):
+    """Favre-averaged u~ of the three probe stations: (array (3, HIG), records used)."""
+    ru = np.zeros((3, HIG))
+    n = C.c_uint()
+    rc = load().fsl_favre_average(path.encode(), HIG, step_start, step_end, real_bytes, _ptr(ru), C.byref(n))
+    if rc:
+        raise FslError("fsl_favre_average (%d)" % rc)
+    return ru, n.value
+
+
+def favre_write_profiles(dirname, ru):
+    ru = np.ascontiguousarray(ru, dtype=np.float64)
+    rc = load().fsl_favre_write_profiles(dirname.encode(), ru.shape[1], _ptr(ru))
+    if rc:
+        raise FslError("fsl_favre_write_profiles (%d)" % rc)
 
 
 def shear_statistics(case, prof, U1, U2, rho0=1.0):

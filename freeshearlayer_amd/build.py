@@ -39,8 +39,8 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-std=c99", "-Wall", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE]
 
 HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip",
-               "fsl_mlp.hip", "fsl_learned.hip"]
-HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c"]
+               "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip"]
+HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c", "fsl_files.c"]
 
 
 def _run(cmd, cwd=None):
@@ -77,7 +77,7 @@ def build_native(jobs=8, force=True, verbose=False):
         obj = os.path.join(BUILD, s.replace(".hip", ".o"))
         if force or not _newer(obj, [src] + hdrs):
             tasks.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
-    for s in HOST_SOURCES + ["fsl_layer2_api.c", "fsl_layer2_main.c"]:
+    for s in HOST_SOURCES + ["fsl_layer2_api.c", "fsl_layer2_main.c", "fsl_layer2_p.c"]:
         src = os.path.join(HOST, s)
         obj = os.path.join(BUILD, s.replace(".c", ".o"))
         if force or not _newer(obj, [src] + hdrs):
@@ -101,7 +101,28 @@ def build_native(jobs=8, force=True, verbose=False):
     _run([CC, "-o", exe + ".tmp", os.path.join(BUILD, "fsl_layer2_main.o"), "-L" + LIB, "-lfsl",
           "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(exe + ".tmp", exe)
+    # the Favre-averaging post-processor (favre-average-src/layer2_p.c)
+    exe = os.path.join(LIB, "fsl_layer2_p")
+    _run([CC, "-o", exe + ".tmp", os.path.join(BUILD, "fsl_layer2_p.o"), "-L" + LIB, "-lfsl",
+          "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(exe + ".tmp", exe)
     return libfsl
+
+
+def build_test_programs():
+    """Host programs of the test suite, linked against the built libraries
+    (tests/_bin/, git-ignored): layer2_dropin drives libfsl_layer2.so in the
+    reference's call order."""
+    tdir = os.path.join(ROOT, "tests")
+    src = os.path.join(tdir, "layer2_dropin.c")
+    if not os.path.exists(src):
+        return
+    out = os.path.join(tdir, "_bin")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(out, "layer2_dropin")
+    _run([CC] + C_FLAGS + [src, "-o", exe + ".tmp", "-L" + LIB, "-lfsl_layer2", "-lfsl",
+                           "-Wl,-rpath,$ORIGIN/../../freeshearlayer_amd/lib", "-lm"])
+    os.replace(exe + ".tmp", exe)
 
 
 def build_oracle(with_reference=True):
@@ -113,6 +134,7 @@ def build_oracle(with_reference=True):
 
 def build(jobs=8, oracle=True, verbose=False):
     lib = build_native(jobs=jobs, force=True, verbose=verbose)
+    build_test_programs()
     if oracle:
         build_oracle()
     return lib

@@ -233,6 +233,14 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         delete c;
         return FSL_ERR_ARG;
     }
+    if (c->cfg.fused && cs->nStages != 3) {
+        // the fused stage kernels implement the three-stage TVD RK only; the
+        // reference's nStages = 2 (two-stage update falling through into the
+        // three-stage one, src/evolution.c:15-27) runs on the unfused path
+        fprintf(stderr, "fsl_create: the fused path needs nStages = 3 (case has %d)\n", cs->nStages);
+        delete c;
+        return FSL_ERR_UNSUPPORTED;
+    }
     const int L = cs->LEN, H = cs->HIG, D = cs->DEP;
     if (L < 1 || H < 1 || D < 1) {
         fprintf(stderr, "fsl_create: mesh %d x %d x %d has an empty direction\n", L, H, D);
@@ -281,7 +289,7 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
         if (c->cfg.fused) {
             if (hipMalloc(&c->Uq, sb) != hipSuccess) { rc = set_err(c, FSL_ERR_NOMEM, "Uq"); break; }
             c->bytes += sb;
-            (void)hipMemsetAsync(c->Uq, 0, sb, c->stream);
+            if (hipMemsetAsync(c->Uq, 0, sb, c->stream) != hipSuccess) { rc = set_err(c, FSL_ERR_HIP, "clear Uq"); break; }
         }
         if (hipMalloc(&c->mu, c->g.G * sizeof(double)) != hipSuccess ||
             hipMalloc(&c->u_in, (size_t)H * sizeof(double)) != hipSuccess ||
@@ -296,9 +304,14 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
             rc = set_err(c, FSL_ERR_NOMEM, "sums");
             break;
         }
-        (void)hipMemsetAsync(c->U, 0, sb, c->stream);
-        (void)hipMemsetAsync(c->Up, 0, sb, c->stream);
-        (void)hipMemsetAsync(c->mu, 0, c->g.G * sizeof(double), c->stream);
+        // ghost layers and mu_SGS read as 0 until written (the dynamic model
+        // and the network's features read U's ghosts, the fluxes mu's)
+        if (hipMemsetAsync(c->U, 0, sb, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->Up, 0, sb, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->mu, 0, c->g.G * sizeof(double), c->stream) != hipSuccess) {
+            rc = set_err(c, FSL_ERR_HIP, "clearing the state buffers failed");
+            break;
+        }
         // reference inflow: 76.4 m/s for rows j <= HIG/2, 200 above
         std::vector<double> u(H);
         for (int j = 1; j <= H; j++) u[j - 1] = (j <= H / 2) ? 76.4 : 200;
@@ -343,6 +356,7 @@ void fsl_destroy(fsl_ctx* c)
     (void)hipFree(c->u_in);
     (void)hipFree(c->sums);
     (void)hipFree(c->red);
+    (void)hipFree(c->probe_buf);
     if (c->red_host) (void)hipHostFree(c->red_host);
     learned_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -640,17 +654,16 @@ int fsl_check_co_num(fsl_ctx* c, double* maxCo_out)
     double m;
     memcpy(&m, &bits, sizeof m);
     if (maxCo_out) *maxCo_out = m;
-    if (mode == 0) {
-        // src/helpers.c:47-50
-        c->dt *= (c->cs.maxCoNum / m);
-    } else {
-        c->dt = c->cfg.cfl / m;
-    }
+    // the reduction carries NaN cells through, so a partly non-finite state
+    // gives NaN here; the context keeps its last valid time step
+    const double dt = (mode == 0) ? c->dt * (c->cs.maxCoNum / m)   // src/helpers.c:47-50
+                                  : c->cfg.cfl / m;
+    if (!(m > 0.0) || !isfinite(m) || !isfinite(dt) || !(dt > 0.0))
+        return set_err(c, FSL_ERR_NONFINITE, "CFL reduction gave %g (dt %g)", m, dt);
+    c->dt = dt;
     c->dtX = c->dt / c->cs.deltaX;
     c->dtY = c->dt / c->cs.deltaY;
     c->dtZ = c->dt / c->cs.deltaZ;
-    if (!(m > 0.0) || !isfinite(m) || !isfinite(c->dt))
-        return set_err(c, FSL_ERR_NONFINITE, "CFL reduction gave %g (dt %g)", m, c->dt);
     return FSL_OK;
 }
 
@@ -732,7 +745,11 @@ int fsl_copy_kernel(fsl_ctx* c, size_t n, int reps)
         (void)hipFree(a);
         return set_err(c, FSL_ERR_NOMEM, "copy buffer");
     }
-    (void)hipMemsetAsync(a, 0, n * sizeof(double), c->stream);
+    if (hipMemsetAsync(a, 0, n * sizeof(double), c->stream) != hipSuccess) {
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return set_err(c, FSL_ERR_HIP, "clear copy buffer");
+    }
     for (int r = 0; r < reps; r++) {
         prof_mark(c, K_COPY, 0);
         launch_copy(c->stream, a, b, n);

@@ -70,6 +70,7 @@ struct fsl_ctx {
     double* sums;       // partials + 5 results
     unsigned long long* red;
     unsigned long long* red_host;   // pinned copy of the CFL maximum (hipHostMalloc)
+    double* probe_buf;  // 6 HIG probe values (allocated on first use)
     size_t bytes;
 
     int cur_is_p;       // stage input: U (0) or Up (1)

@@ -366,16 +366,19 @@ __device__ __forceinline__ double cfl_cell(int mode, const double q[5], const Ph
     return (fabs(u) + cs) * ph.rdel[0] + (fabs(vv) + cs) * ph.rdel[1] + (fabs(w) + cs) * ph.rdel[2];
 }
 
-// The reference seeds the maximum with 0 and skips NaN (comparison false):
-// every lane contributes max(+0, value), and the maximum is folded into a
-// 64-bit atomic max on the bit pattern, which orders non-negative doubles.
+// Running maximum of the CFL quantity.  The reference seeds the maximum with
+// 0 and its comparison skips NaN cells (src/helpers.c:39-41); here a NaN
+// cell makes the maximum NaN, so a partly non-finite state cannot give a
+// finite time step.  Finite maxima are the reference's, bit for bit.
+__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
+
+// Every lane contributes max(+0, value) and the wave maximum is folded into
+// a 64-bit atomic max on the bit pattern, which orders non-negative doubles
+// and puts a NaN (exponent all ones, non-zero mantissa) above +inf.
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = (w > v) ? w : v;
-    }
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     return v;
 }
 

@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 }
                 if (STAGE == 3 && a.cfl_out) {
                     const double v = mx::cfl_cell(a.cfl_mode, ov, a.ph, a.dtX);
-                    cfl_m = (v > cfl_m) ? v : cfl_m;
+                    cfl_m = nan_max(cfl_m, v);
                 }
             }
 #pragma unroll

@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_cfl(CState5 U, Geo g, Phys ph, doubl
             const double cs = sqrt(kK * p / R);
             v = (fabs(u) + cs) * ph.rdel[0] + (fabs(vv) + cs) * ph.rdel[1] + (fabs(w) + cs) * ph.rdel[2];
         }
-        m = (v > m) ? v : m;
+        m = nan_max(m, v);
     }
     m = wave_max(m);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(kBlock) k_cfl(CState5 U, Geo g, Phys ph, doubl
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = red[0];
-        for (int w = 1; w < kBlock / 64; w++) b = (red[w] > b) ? red[w] : b;
+        for (int w = 1; w < kBlock / 64; w++) b = nan_max(b, red[w]);
         atomicMax(out, (unsigned long long)__double_as_longlong(b));
     }
 }

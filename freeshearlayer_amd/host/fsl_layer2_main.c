@@ -5,10 +5,19 @@
  * reads layer2.ini (or an mpi_layer2.bin), the LCG seed from stdin, starts
  * from the two-stream initial condition or from backup.dat (Answer = 1),
  * advances numstep steps, stops early on SIGINT or on a stopfile holding 's'
- * (src-par/mpi_layer2.c:169-191), and writes backup.dat.
+ * (src-par/mpi_layer2.c:169-191), appends the probes record of every step to
+ * probes.dat (src/probes.c), writes the Tecplot frame <step>.plt every f_step
+ * steps and at the end (src/output.c), and writes backup.dat (src/finalize.c)
+ * -- the frame and backup.dat not after a failed step.  stdout is
+ * line-buffered so a supervisor sees each step's line.
  *
  *   fsl_layer2 [-i layer2.ini | -b mpi_layer2.bin] [-n steps] [--const-cs]
- *              [--fused] [--mixed] [--acoustic CFL] [--learned MODEL] [--real4] [-q] < seed
+ *              [--fused] [--mixed] [--acoustic CFL] [--learned MODEL] [--real4]
+ *              [--no-output] [-q] < seed
+ *
+ * --real4 writes probes.dat and backup.dat with 4-byte reals (the layout of
+ * the shipped float build, which the Favre post-processor reads);
+ * --no-output skips probes.dat and the frames.
  *
  * --learned MODEL runs the learned SGS closure (parameter file as written by
  * fsl_mlp_file_write), judged against the dynamic model every 50 steps.
@@ -42,7 +51,7 @@ int main(int argc, char **argv)
 {
     const char *ini = "layer2.ini", *bin = 0, *model = 0;
     long nsteps = -1;
-    int quiet = 0, real_bytes = 8, a, rc;
+    int quiet = 0, real_bytes = 8, output = 1, a, rc;
     double X = 0.0, total = 0.0;
     fsl_case c;
     fsl_config cfg;
@@ -50,6 +59,7 @@ int main(int argc, char **argv)
     double *U;
     size_t G;
 
+    setvbuf(stdout, 0, _IOLBF, 0);
     fsl_config_defaults(&cfg);
     for (a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "-i") && a + 1 < argc) ini = argv[++a];
@@ -65,10 +75,11 @@ int main(int argc, char **argv)
             cfg.sgs_model = FSL_SGS_LEARNED;
             model = argv[++a];
         } else if (!strcmp(argv[a], "--real4")) real_bytes = 4;
+        else if (!strcmp(argv[a], "--no-output")) output = 0;
         else if (!strcmp(argv[a], "-q")) quiet = 1;
         else {
             fprintf(stderr, "usage: %s [-i ini | -b bin] [-n steps] [--const-cs] [--fused] [--mixed] "
-                            "[--acoustic CFL] [--learned MODEL] [--real4] [-q] < seed\n", argv[0]);
+                            "[--acoustic CFL] [--learned MODEL] [--real4] [--no-output] [-q] < seed\n", argv[0]);
             return 2;
         }
     }
@@ -126,8 +137,13 @@ int main(int argc, char **argv)
         rc = fsl_step(ctx);
         total += fsl_get_dt(ctx);
         if (!quiet) printf("Timestep no.: %u, total time: %f sec.\n", fsl_get_step(ctx), total);
+        if (rc == FSL_OK && output) {   /* src/layer2.c:180-183 */
+            rc = fsl_write_probes(ctx, "probes.dat", real_bytes);
+            if (rc == FSL_OK && c.f_step && fsl_get_step(ctx) % c.f_step == 0) rc = fsl_write_plt(ctx, 0);
+        }
         if (stopfile_says_stop()) break;
     }
+    if (rc == FSL_OK && output) rc = fsl_write_plt(ctx, 0);   /* src/layer2.c:187-188 */
     if (rc != FSL_OK) fprintf(stderr, "step failed: %s\n", fsl_last_error(ctx));
     if (model && !quiet) {
         fsl_learned_status st;
@@ -135,8 +151,14 @@ int main(int argc, char **argv)
         printf("Learned closure: %lld steps on the network, %lld on the dynamic model, %lld judges, "
                "%lld fallbacks\n", st.learned_steps, st.dynamic_steps, st.judged, st.fallbacks);
     }
-    if (fsl_download_state(ctx, 0, U) == FSL_OK)
+    /* src/finalize.c:14-34 writes backup.dat at the end of the run.  A run
+       that ended on a failed step (e.g. a non-finite CFL reduction) leaves
+       the previous backup.dat -- with Answer = 1 the run's own input --
+       untouched instead of overwriting it with the diverged state. */
+    if (rc == FSL_OK && fsl_download_state(ctx, 0, U) == FSL_OK)
         fsl_backup_write("backup.dat", &c, U, fsl_get_dt(ctx), fsl_get_step(ctx), real_bytes);
+    else if (rc != FSL_OK)
+        fprintf(stderr, "backup.dat not written: the last step failed\n");
     fsl_destroy(ctx);
     free(U);
     puts("Simulation finished!");

@@ -273,6 +273,44 @@ typedef struct fsl_shear_stats {
 int fsl_shear_statistics(const fsl_case *c, const double *prof, double U1, double U2, double rho0,
                          double *dtheta_x, double *dw_x, fsl_shear_stats *out);
 
+/* --- output files of the reference (src/output.c, src/probes.c,
+       favre-average-src/layer2_p.c), from the device step state --- */
+/* The nine computed columns of the Tecplot frame of src/output.c:88-136 for
+   the interior planes k0 .. k0 + nk - 1 (1-based): rho, u, v, w, p, T,
+   vorticity magnitude, Q = (|omega|^2 - |S|^2)/2 and mu_T / mu_L (the
+   constant-coefficient rho CsDD |S| / mu_L, as src/output.c:134 writes in
+   every SGS mode), FP64 in the reference's operation order; out holds
+   9 values per cell, cells in the frame's order (i fastest, then j, k),
+   nk LEN HIG cells. */
+int fsl_output_fields(fsl_ctx *ctx, int k0, int nk, double *out);
+/* Replaces Output() (src/output.c:16-143): writes the frame of the step
+   state to path (NULL: "<step>.plt" in the working directory, as the
+   reference names it), formatted with the reference's "%g" conversions. */
+int fsl_write_plt(fsl_ctx *ctx, const char *path);
+/* The same frame from host fields (9 values per interior cell in the
+   frame's order, LEN HIG DEP cells, as fsl_output_fields gives them). */
+int fsl_plt_write_fields(const char *path, const fsl_case *c, const double *fields);
+/* Probe columns of src/probes.c:20-40: rho and rho u along j = 1..HIG at
+   i = LEN/2, 3 LEN/4 and LEN on the plane k = DEP/2 + 1, in that order
+   (6 HIG values: rho, rho u of station 1, then 2, then 3). */
+int fsl_probe_values(fsl_ctx *ctx, double *out);
+/* Replaces Probes() (src/probes.c:13-43): appends the step (uint32) and the
+   six columns as `real` (real_bytes 8: double, 4: float as shipped) to path
+   (NULL: "probes.dat"). */
+int fsl_write_probes(fsl_ctx *ctx, const char *path, int real_bytes);
+int fsl_probes_append(const char *path, unsigned step, unsigned HIG, const double *vals, int real_bytes);
+/* favre-average-src/layer2_p.c:27-113: from a probes file, the sums of rho
+   and rho u over the records with step_start <= step <= step_end (float
+   arithmetic for the 4-byte layout, as the post-processor does), then the
+   Favre mean u~ = sum(rho u) / sum(rho) of each station and row:
+   ru[s HIG + j], s = 0..2.  Reading stops at the first record past step_end
+   or at the end of the file; nrec (may be NULL) receives the records used. */
+int fsl_favre_average(const char *path, unsigned HIG, unsigned step_start, unsigned step_end, int real_bytes,
+                      double *ru, unsigned *nrec);
+/* favre-average-src/layer2_p.c:115-147: probe1.dat .. probe3.dat in dir
+   (NULL or "": the working directory). */
+int fsl_favre_write_profiles(const char *dir, unsigned HIG, const double *ru);
+
 /* --- learned SGS closure (sgs_model = FSL_SGS_LEARNED) --- */
 /* Network 8 -> 64 -> 64 -> 1 with tanh hidden layers; BF16 operands on the
    matrix cores with FP32 accumulation.  Output y gives Cd = clip(0.15 y, 0,

@@ -819,6 +819,68 @@ void or_sim_destroy(or_sim *m)
     free(m);
 }
 
+/* src/output.c:88-136 (with real = double): primitives, central
+   differences of u = rho u / rho with the 1/(2 delta) factors, vorticity,
+   strain magnitude, Q and the constant-coefficient mu_T / mu_L ratio, in the
+   reference's association order. */
+void or_output_fields(const or_params *p, double *const U[5], double *out)
+{
+    const int L = p->L, H = p->H, D = p->D;
+    const long sj = D + 2, si = (long)(H + 2) * (D + 2);
+    int i, j, k;
+    double *o = out;
+    for (k = 1; k <= D; k++)
+        for (j = 1; j <= H; j++)
+            for (i = 1; i <= L; i++, o += 9) {
+                const long c = i * si + j * sj + k;
+                const double *r = U[0], *mu = U[1], *mv = U[2], *mw = U[3];
+                double R, Uc, Vc, Wc, P, T, du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz;
+                double ox, oy, oz, Omega, S12, S13, S23, Strain, Q, muT;
+                R = r[c];
+                Uc = mu[c] / R;
+                Vc = mv[c] / R;
+                Wc = mw[c] / R;
+                P = (U[4][c] - 0.5 * R * (Uc * Uc + Vc * Vc + Wc * Wc)) * 0.4;
+                T = P / (287.10 * R);
+                du_dx = (mu[c + si] / r[c + si] - mu[c - si] / r[c - si]) * p->r2dx;
+                du_dy = (mu[c + sj] / r[c + sj] - mu[c - sj] / r[c - sj]) * p->r2dy;
+                du_dz = (mu[c + 1] / r[c + 1] - mu[c - 1] / r[c - 1]) * p->r2dz;
+                dv_dx = (mv[c + si] / r[c + si] - mv[c - si] / r[c - si]) * p->r2dx;
+                dv_dy = (mv[c + sj] / r[c + sj] - mv[c - sj] / r[c - sj]) * p->r2dy;
+                dv_dz = (mv[c + 1] / r[c + 1] - mv[c - 1] / r[c - 1]) * p->r2dz;
+                dw_dx = (mw[c + si] / r[c + si] - mw[c - si] / r[c - si]) * p->r2dx;
+                dw_dy = (mw[c + sj] / r[c + sj] - mw[c - sj] / r[c - sj]) * p->r2dy;
+                dw_dz = (mw[c + 1] / r[c + 1] - mw[c - 1] / r[c - 1]) * p->r2dz;
+                ox = dw_dy - dv_dz;
+                oy = du_dz - dw_dx;
+                oz = dv_dx - du_dy;
+                Omega = sqrt(ox * ox + oy * oy + oz * oz);
+                S12 = 0.5 * (du_dy + dv_dx);
+                S13 = 0.5 * (du_dz + dw_dx);
+                S23 = 0.5 * (dv_dz + dw_dy);
+                Strain = sqrt(2 * (du_dx * du_dx + dv_dy * dv_dy + dw_dz * dw_dz +
+                                   2 * (S12 * S12 + S13 * S13 + S23 * S23)));
+                Q = 0.5 * (Omega * Omega - Strain * Strain);
+                muT = R * p->CsDD * Strain / p->mu_L;
+                o[0] = R; o[1] = Uc; o[2] = Vc; o[3] = Wc; o[4] = P; o[5] = T;
+                o[6] = Omega; o[7] = Q; o[8] = muT;
+            }
+}
+
+/* src/probes.c:17-40: stations i = LEN/2, 3*LEN/4, LEN (unsigned
+   arithmetic), mid-z plane k = DEP/2 + 1 */
+void or_probe_values(const or_params *p, double *const U[5], double *out)
+{
+    const unsigned L = (unsigned)p->L, H = (unsigned)p->H, D = (unsigned)p->D;
+    const long sj = D + 2, si = (long)(H + 2) * (D + 2);
+    const unsigned st[3] = {L / 2, 3 * L / 4, L};
+    const unsigned k = D / 2 + 1;
+    unsigned s, f, j;
+    for (s = 0; s < 3; s++)
+        for (f = 0; f < 2; f++)
+            for (j = 1; j <= H; j++) *out++ = U[f][st[s] * si + j * sj + k];
+}
+
 #define OR_KEYS(X_) \
     X_(dx) X_(dy) X_(dz) X_(dt) X_(mu_L) X_(Pr_L) X_(Pr_T) X_(Cs) X_(Ua) X_(Va) X_(Wa) \
     X_(maxCoNum) X_(rho_in) X_(p_in) X_(p_out) X_(rdx) X_(rdy) X_(rdz) X_(r2dx) X_(r2dy) \
@@ -896,3 +958,6 @@ void or_sim_step(or_sim *m) { or_step(&m->p, &m->s); }
 void or_sim_ic_reference(or_sim *m) { or_ic_reference(&m->p, &m->s); }
 int or_sim_cur_is_p(or_sim *m) { return m->s.cur_is_p; }
 void or_sim_set_cur(or_sim *m, int is_p) { m->s.cur_is_p = is_p; }
+
+void or_sim_output_fields(or_sim *m, double *out) { or_output_fields(&m->p, m->s.U, out); }
+void or_sim_probe_values(or_sim *m, double *out) { or_probe_values(&m->p, m->s.U, out); }

@@ -17,6 +17,8 @@
  *   src/helpers.c:13-17,23-52              -> or_random, or_check_co_num
  *   src/layer2.c:141-185                   -> or_step (stage driver, block draw)
  *   src/initialize.c:62-94                 -> or_params_derive
+ *   src/output.c:88-136                    -> or_output_fields
+ *   src/probes.c:17-40                     -> or_probe_values
  *
  * Every function here is pinned by tests/test_oracle_*.py: bit-for-bit against
  * the compiled reference (oracle/_ref, built from /root/reference/src by
@@ -113,6 +115,14 @@ void or_dynamic_smagorinsky(const or_params *p, double *const U[5], double *mu_s
 /* One time step in the order of src/layer2.c:141-185 (probes/output left out). */
 void or_step(or_params *p, or_state *s);
 
+/* The nine computed columns of Output()'s Tecplot rows (src/output.c:88-136:
+   rho, u, v, w, p, T, vorticity magnitude, Q, rho CsDD |S| / mu_L) of the
+   step state for every interior cell, i fastest, then j, then k
+   (9 L H D values). */
+void or_output_fields(const or_params *p, double *const U[5], double *out);
+/* Probes() columns (src/probes.c:17-40): U1 and U2 along j = 1..H at
+   i = L/2, 3L/4, L on k = D/2 + 1 (6 H values, in the file's order). */
+void or_probe_values(const or_params *p, double *const U[5], double *out);
 /* Fault-injection hook for tests only: when nonzero, or_fluxes drops the heat
    flux term.  Used to show the parity tests catch a dropped term. */
 extern int or_fault_drop_heat_flux;
@@ -142,6 +152,8 @@ void    or_sim_step(or_sim *s);
 void    or_sim_ic_reference(or_sim *s);
 int     or_sim_cur_is_p(or_sim *s);
 void    or_sim_set_cur(or_sim *s, int is_p);
+void    or_sim_output_fields(or_sim *s, double *out);
+void    or_sim_probe_values(or_sim *s, double *out);
 
 #ifdef __cplusplus
 }

@@ -13,8 +13,9 @@
  * sets DynamicSmagorinskySGS to 0 after the reference's def.h.  The output goes
  * only to oracle/_ref/.  Nothing of the reference is copied into this tree.
  *
- * The harness replays main()'s per-step order (src/layer2.c:141-185) and
- * exposes the reference's global arrays to Python through ctypes.
+ * The harness replays main()'s per-step order (src/layer2.c:141-185),
+ * exposes the reference's global arrays to Python through ctypes, and runs
+ * the reference's own main (fslref_main) for the output-file fixtures.
  */
 #include <fcntl.h>
 #include <stdio.h>
@@ -252,6 +253,26 @@ API void fslref_block_draw(void)
     X = Random(X); Ud = Ua * (1 - X - X);
     X = Random(X); Vd = Va * (1 - X - X);
     X = Random(X); Wd = Wa * (1 - X - X);
+}
+
+/* The reference's own main (src/layer2.c:117-195, renamed by the build):
+   with dir as working directory (layer2.ini there; probes.dat, <step>.plt
+   and backup.dat written there) and the LCG seed read from seed_path as its
+   stdin.  For a fresh process: main's globals are not reset afterwards. */
+int fslref_layer2_main(void);
+
+API int fslref_main(const char *dir, const char *seed_path)
+{
+    char cwd[4096];
+    int rc;
+    if (!getcwd(cwd, sizeof cwd)) return -1;
+    if (!freopen(seed_path, "r", stdin)) return -2;
+    if (chdir(dir) != 0) return -3;
+    mute();
+    rc = fslref_layer2_main();
+    unmute();
+    if (chdir(cwd) != 0) return -4;
+    return rc;
 }
 
 /* one time step in main()'s order, src/layer2.c:141-185 (no probes/output) */

@@ -24,21 +24,35 @@ Fixtures
   g5_filter.npz                    filter_ of a seeded field, 9x7x5
   g6_nstages2.npz                  16x12x8 with nStages = 2 (Evolution falls
                                    through), 4 steps
+  g7_<v>_5.plt.gz, g7_<v>_probes.dat   (v = cs, dyn) the reference's own main
+                                   (src/layer2.c) on 16x12x8, 5 steps, f_step 3,
+                                   seed X0: its Output() frame after step 5 and
+                                   the probes.dat its Probes() appended (8-byte
+                                   reals: real = double)
+  g7_<v>_probes_f32.dat            the same records in the 4-byte layout of the
+                                   shipped build (converted here)
+  g7_<v>_probe{1,2,3}.dat          the reference's Favre post-processor
+                                   (favre-average-src/layer2_p.c) on that file,
+                                   stepStart 2, stepEnd 4, HIG 12
   case_layer2.ini, case_mpi_layer2.bin, case_ini2bin.ini, case_ini2bin.bin,
   case_seed.txt                    the reference's case files (data)
 """
+import ctypes
+import gzip
 import hashlib
 import json
 import os
 import shutil
+import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from oracle_lib import RefLib  # noqa: E402
+from oracle_lib import REF_DIR, RefLib, write_ini  # noqa: E402
 from fsl_test_util import random_state  # noqa: E402
 
 REF_ROOT = "/root/reference"
@@ -160,7 +174,62 @@ def g6():
     return out
 
 
+G7 = dict(L=16, H=12, D=8, numstep=5, f_step=3, BL_HIG=5, nStages=3)
+
+
+def run_reference_main(variant, wd):
+    """The reference's main in this (fresh) process, working directory wd."""
+    lib = ctypes.CDLL(os.path.join(REF_DIR, "libfslref_%s.so" % variant))
+    lib.fslref_main.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    seed = os.path.join(wd, "seed.txt")
+    with open(seed, "w") as f:
+        f.write("%.12f\n" % X0)
+    return lib.fslref_main(wd.encode(), seed.encode())
+
+
+def probes_to_f32(raw, H):
+    """probes.dat records (uint32 step + 6 H doubles) in the float layout."""
+    rec = 4 + 6 * H * 8
+    out = b""
+    for o in range(0, len(raw), rec):
+        out += raw[o:o + 4] + np.frombuffer(raw[o + 4:o + rec], dtype=np.float64).astype(np.float32).tobytes()
+    return out
+
+
+def g7(variant):
+    """Output files of the reference's own main and post-processor."""
+    files = {}
+    L, H, D = G7["L"], G7["H"], G7["D"]
+    with tempfile.TemporaryDirectory() as wd:
+        write_ini(os.path.join(wd, "layer2.ini"), L, H, D, numstep=G7["numstep"], f_step=G7["f_step"],
+                  BL_HIG=G7["BL_HIG"], nStages=G7["nStages"])
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--run-main", variant, wd],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError("reference main failed: " + r.stdout[-500:])
+        assert os.path.exists(os.path.join(wd, "3.plt"))
+        files["g7_%s_5.plt.gz" % variant] = gzip.compress(open(os.path.join(wd, "5.plt"), "rb").read(), mtime=0)
+        raw = open(os.path.join(wd, "probes.dat"), "rb").read()
+        assert len(raw) == G7["numstep"] * (4 + 6 * H * 8)
+        files["g7_%s_probes.dat" % variant] = raw
+        f32 = probes_to_f32(raw, H)
+        files["g7_%s_probes_f32.dat" % variant] = f32
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "probes.dat"), "wb").write(f32)
+        # stepEnd below the last record: the post-processor ends its loop on the
+        # first record past stepEnd (it has no end-of-file test)
+        r = subprocess.run([os.path.join(REF_DIR, "layer2_p")], cwd=wd, input="2\n4\n%d\n" % H,
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+        if r.returncode != 0:
+            raise RuntimeError("layer2_p failed: " + r.stdout[-500:])
+        for q in (1, 2, 3):
+            files["g7_%s_probe%d.dat" % (variant, q)] = open(os.path.join(wd, "probe%d.dat" % q), "rb").read()
+    return files
+
+
 def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--run-main":
+        return run_reference_main(sys.argv[2], sys.argv[3])
     if not os.path.isdir(REF_ROOT):
         print("reference not present; nothing to do")
         return 1
@@ -182,6 +251,13 @@ def main():
         manifest[name + ".npz"] = {"sha256": hashlib.sha256(open(path, "rb").read()).hexdigest(),
                                    "keys": sorted(data.keys())}
         print("wrote", path, os.path.getsize(path))
+    for variant in ("cs", "dyn"):
+        if only and "g7_" + variant not in only:
+            continue
+        for name, data in g7(variant).items():
+            open(os.path.join(HERE, name), "wb").write(data)
+            manifest[name] = {"sha256": hashlib.sha256(data).hexdigest(), "case": G7, "seed": X0}
+            print("wrote", name, len(data))
     for src, dst in (("layer2.ini", "case_layer2.ini"), ("mpi_layer2.bin", "case_mpi_layer2.bin"),
                      ("src-par/ini2bin/mpi_layer2.ini", "case_ini2bin.ini"),
                      ("src-par/ini2bin/mpi_layer2.bin", "case_ini2bin.bin"), ("file", "case_seed.txt")):

@@ -68,6 +68,8 @@ class Oracle:
             lib.or_sim_check_co_num.argtypes = [C.c_void_p]
             lib.or_sim_check_co_num.restype = C.c_double
             lib.or_sim_filter.argtypes = [C.c_void_p, _dp, _dp]
+            lib.or_sim_output_fields.argtypes = [C.c_void_p, _dp]
+            lib.or_sim_probe_values.argtypes = [C.c_void_p, _dp]
             lib.or_sim_cur_is_p.argtypes = [C.c_void_p]
             lib.or_sim_cur_is_p.restype = C.c_int
             lib.or_sim_set_cur.argtypes = [C.c_void_p, C.c_int]
@@ -138,6 +140,17 @@ class Oracle:
     def ic_reference(self): self.lib().or_sim_ic_reference(self.h)
     def cur_is_p(self): return self.lib().or_sim_cur_is_p(self.h)
     def set_cur(self, is_p): self.lib().or_sim_set_cur(self.h, int(is_p))
+
+    def output_fields(self):
+        """Output()'s nine computed columns per interior cell: (D, H, L, 9)."""
+        out = np.zeros((self.D, self.H, self.L, 9))
+        self.lib().or_sim_output_fields(self.h, _ptr(out))
+        return out
+
+    def probe_values(self):
+        out = np.zeros((6, self.H))
+        self.lib().or_sim_probe_values(self.h, _ptr(out))
+        return out
 
     def filter(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64)

@@ -33,3 +33,16 @@ def test_error_codes_match_the_header():
     hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "fsl.h")).read()
     codes = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define (FSL_ERR_\w+)\s+(-?\d+)", hdr))
     assert codes and all(getattr(fsl, k) == v for k, v in codes.items()), codes
+
+
+def test_fused_path_rejects_two_stages():
+    """The fused stage kernels implement the three-stage TVD RK only: the
+    reference's nStages = 2 (its two-stage update falling through into the
+    three-stage one, src/evolution.c:15-27) is an unfused-path case, and
+    fsl_create refuses the fused path for it before any device call."""
+    case = fsl.case_reference(16, 12, 8, nStages=2)
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
+        fsl.Context(case, fsl.config(fused=1))
+    for n in (1, 4):   # not a stage count of the reference (src/evolution.c:15-27)
+        with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_ARG):
+            fsl.Context(fsl.case_reference(16, 12, 8, nStages=n), fsl.config(fused=0))

@@ -66,3 +66,80 @@ def test_learned_closure_from_the_command_line(tmp_path):
     r = subprocess.run([EXE, "--learned", str(wd / "missing.bin")], cwd=str(wd), input="0.3\n",
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 1 and "cannot load the learned closure" in r.stderr
+
+
+def _backup_step(path):
+    raw = open(path, "rb").read()
+    return int(np.frombuffer(raw[-4:], dtype=np.uint32)[0]), raw
+
+
+def test_stopfile_ends_the_run_after_the_current_step(tmp_path):
+    """src-par/mpi_layer2.c:169-191: a stopfile holding 's' stops the run
+    after the step in progress; backup.dat is written for that step."""
+    wd = tmp_path / "stop"
+    wd.mkdir()
+    write_ini(str(wd / "layer2.ini"), L, H, D, numstep=1000, f_step=1000, BL_HIG=5)
+    (wd / "stopfile").write_text("s\n")
+    r = subprocess.run([EXE, "-q"], cwd=str(wd), input="0.317391239817\n", stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0 and "Simulation finished!" in r.stdout
+    step, _ = _backup_step(wd / "backup.dat")
+    assert step == 1
+    (wd / "stopfile").write_text("c\n")   # anything else does not stop it
+    write_ini(str(wd / "layer2.ini"), L, H, D, numstep=3, f_step=1000, BL_HIG=5)
+    r = subprocess.run([EXE, "-q"], cwd=str(wd), input="0.317391239817\n", stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0 and _backup_step(wd / "backup.dat")[0] == 3
+
+
+def test_sigint_ends_the_run_after_the_current_step(tmp_path):
+    """src/layer2.c:107-127: SIGINT clears the continue flag; the loop ends
+    after the current step and the run still writes backup.dat."""
+    import signal
+    import time
+    wd = tmp_path / "sigint"
+    wd.mkdir()
+    write_ini(str(wd / "layer2.ini"), L, H, D, numstep=10 ** 7, f_step=1000, BL_HIG=5)
+    p = subprocess.Popen([EXE], cwd=str(wd), stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    try:
+        p.stdin.write("0.317391239817\n")
+        p.stdin.close()
+        seen = 0
+        t0 = time.time()
+        while time.time() - t0 < 60:
+            line = p.stdout.readline()
+            if line.startswith("Timestep no.:"):
+                seen += 1
+                if seen == 3:
+                    break
+        assert seen == 3, "the run printed no step lines"
+        p.send_signal(signal.SIGINT)
+        p.wait(timeout=60)
+        out, err = p.stdout.read(), p.stderr.read()
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    assert p.returncode == 0 and "Simulation finished!" in out, err
+    step, _ = _backup_step(wd / "backup.dat")
+    assert 3 <= step < 10 ** 7
+
+
+def test_failed_step_leaves_backup_dat_unchanged(tmp_path):
+    """A restart (Answer = 1) from a state with a NaN cell fails its first
+    CFL rescale (FSL_ERR_NONFINITE); the run exits 1 and backup.dat -- its
+    own input -- is left byte for byte as it was."""
+    wd = tmp_path / "nan"
+    wd.mkdir()
+    case = fsl.case_reference(L, H, D, BL_HIG=5)
+    U = fsl.ic_reference(case)
+    U[0, 5, 4, 3] = np.nan
+    assert fsl.load().fsl_backup_write(str(wd / "backup.dat").encode(), fsl.C.byref(case), fsl._ptr(U),
+                                       fsl.C.c_double(7e-4), 0, 8) == 0
+    before = open(wd / "backup.dat", "rb").read()
+    write_ini(str(wd / "layer2.ini"), L, H, D, numstep=STEPS, f_step=1000, BL_HIG=5, answer=1)
+    r = subprocess.run([EXE, "-q"], cwd=str(wd), input="0.317391239817\n", stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 1 and "step failed" in r.stderr and "not written" in r.stderr
+    assert open(wd / "backup.dat", "rb").read() == before
