@@ -184,6 +184,7 @@ def load():
         "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
         "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
         "fsl_output_fields": (i_, [vp, i_, i_, _dp]),
+        "fsl_conservation_sums_noalloc": (i_, [vp, u_, u_, u_, vp, vp]),
         "fsl_write_plt": (i_, [vp, cc]),
         "fsl_plt_write_fields": (i_, [cc, C.POINTER(FslCase), _dp]),
         "fsl_probe_values": (i_, [vp, _dp]),

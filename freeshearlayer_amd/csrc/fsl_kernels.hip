@@ -13,6 +13,12 @@
 // coalesced 64-element runs.
 #include "fsl_internal.hpp"
 
+#include <stdio.h>
+
+#include <mutex>
+
+#include "fsl.h"
+
 namespace fsl {
 
 namespace {
@@ -444,4 +450,82 @@ void launch_copy(hipStream_t s, const double* a, double* b, size_t n)
     hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, s, a, b, (long long)n);
 }
 
+// ---------------------------------------------------------------------
+// Allocation-free conservation sums.  The per-block partials of
+// k_sums_partial live in a file-scope device array instead of a context
+// allocation: at most kSumsMaxBlocks blocks x 5 fields (40 KiB), held by
+// every device the library runs on until the process ends.  Each device
+// has its own copy; its address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept here -- the
+// host-side name `scratch` is only a stand-in and never reaches a kernel.
+namespace {
+
+constexpr int kSumsMaxBlocks = 1024;
+constexpr int kScratchDoubles = kSumsMaxBlocks * 5;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+double* g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+
+int scratch_of_current_device(double** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return FSL_ERR_HIP;
+    if (dev < 0 || dev >= kMaxDevices) return FSL_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "fsl_conservation_sums_noalloc: hipGetSymbolAddress failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return FSL_ERR_HIP;
+        }
+        g_scratch[dev] = (double*)p;
+    }
+    *out = g_scratch[dev];
+    return FSL_OK;
+}
+
+}  // namespace
 }  // namespace fsl
+
+extern "C" int fsl_conservation_sums_noalloc(const double* U, unsigned LEN, unsigned HIG, unsigned DEP, double* sums,
+                                             void* stream)
+{
+    using namespace fsl;
+    if (!U || !sums || LEN < 1 || HIG < 1 || DEP < 1) {
+        fprintf(stderr, "fsl_conservation_sums_noalloc: null buffer or empty mesh (%u x %u x %u)\n", LEN, HIG,
+                DEP);
+        return FSL_ERR_ARG;
+    }
+    Geo g;
+    g.L = (int)LEN;
+    g.H = (int)HIG;
+    g.D = (int)DEP;
+    g.sj = g.D + 2;
+    g.si = (long long)(g.H + 2) * (g.D + 2);
+    g.G = (long long)(g.L + 2) * g.si;
+    const int nb = sums_nblocks(g);
+    if (nb > kSumsMaxBlocks) {   // sums_nblocks caps the partial count at the scratch's size
+        fprintf(stderr, "fsl_conservation_sums_noalloc: %d partial blocks exceed the %d of the scratch array\n", nb,
+                kSumsMaxBlocks);
+        return FSL_ERR_UNSUPPORTED;
+    }
+    double* part = nullptr;
+    const int rc = scratch_of_current_device(&part);
+    if (rc != FSL_OK) return rc;
+    CState5 s;
+    for (int f = 0; f < 5; f++) s.f[f] = U + f * g.G;
+    // every partial the final kernel reads is written first by this call, so
+    // nothing in the scratch needs clearing
+    launch_sums((hipStream_t)stream, s, g, part, nb, sums);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "fsl_conservation_sums_noalloc: launch failed: %s\n", hipGetErrorString(e));
+        return FSL_ERR_HIP;
+    }
+    return FSL_OK;
+}

@@ -251,6 +251,20 @@ int fsl_synchronize(fsl_ctx *ctx);
 /* --- diagnostics (device FP64 reductions) --- */
 /* out[0..4]: sums of rho, rho u, rho v, rho w, E over the interior */
 int fsl_conservation_sums(fsl_ctx *ctx, double out[5]);
+/* The same sums without a context and without allocating: no counterpart in
+   the reference; for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   U: device pointer to a state in the context's layout (5 dense cell arrays
+   (LEN+2)(HIG+2)(DEP+2), one after the other); sums: device pointer to 5
+   doubles, written in stream order; stream: a hipStream_t (NULL: the null
+   stream).  The work is queued, not waited for.  The per-block partials live
+   in a library-owned device array of 1024 x 5 doubles (40 KiB per device,
+   held until the process ends); the partial count, min(ceil(N / 256), 1024),
+   always fits.  Calls that overlap on one device share that array and race:
+   serialise them (one stream, or events).  Same kernels and summation order
+   as fsl_conservation_sums, so the results are bitwise equal. */
+int fsl_conservation_sums_noalloc(const double *U, unsigned LEN, unsigned HIG, unsigned DEP, double *sums,
+                                  void *stream);
 /* Plane profiles of the step state: z sums of rho, rho u, rho v, rho w,
    (rho u)^2/rho, (rho v)^2/rho, (rho w)^2/rho for every interior (i, j);
    out[q LEN HIG + (i-1) HIG + (j-1)], q = 0..6 (7 LEN HIG doubles). */
