@@ -39,7 +39,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-std=c99", "-Wall", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE]
 
 HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip",
-               "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip"]
+               "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip", "fsl_sgs_march.hip"]
 HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c", "fsl_files.c"]
 
 

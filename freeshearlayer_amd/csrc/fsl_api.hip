@@ -136,26 +136,16 @@ Faces faces_of(fsl_ctx* c)
 
 int ensure_sgs(fsl_ctx* c)
 {
-    if (c->sgs) return FSL_OK;
-    const size_t n = sgs_scratch_arrays() * (size_t)c->g.G;
-    HIPCHK(c, hipMalloc(&c->sgs, n * sizeof(double)));
+    // the FP64 model keeps its intermediates on chip; the FP32 model (mixed
+    // contexts) has 16 float scratch arrays
+    if (c->sgs || c->cfg.precision != FSL_PREC_MIXED) return FSL_OK;
+    const size_t n = sgs_f32_scratch_bytes(c->g);
+    HIPCHK(c, hipMalloc(&c->sgs, n));
     // the A tensor's ghost layer must read as 0 (src/turbulence.c:369-379
     // writes the interior only, into fresh zeroed arrays)
-    HIPCHK(c, hipMemsetAsync(c->sgs, 0, n * sizeof(double), c->stream));
-    c->bytes += n * sizeof(double);
+    HIPCHK(c, hipMemsetAsync(c->sgs, 0, n, c->stream));
+    c->bytes += n;
     return FSL_OK;
-}
-
-SgsBufs sgs_bufs(fsl_ctx* c)
-{
-    SgsBufs B;
-    double* p = c->sgs;
-    const long long G = c->g.G;
-    for (int a = 0; a < 3; a++) { B.uf[a] = p; p += G; }
-    for (int a = 0; a < 6; a++) { B.Lij[a] = p; p += G; }
-    for (int a = 0; a < 6; a++) { B.Aij[a] = p; p += G; }
-    B.mag = p;
-    return B;
 }
 
 int ensure_filt(fsl_ctx* c)
@@ -574,7 +564,7 @@ int fsl_dynamic_smagorinsky(fsl_ctx* c)
     if (c->cfg.precision == FSL_PREC_MIXED)
         launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, nullptr);
     else
-        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c, nullptr);
+        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, c->g, c->ph, prof_mark, c, nullptr);
     return check_launch(c, "dynamic_smagorinsky");
 }
 

@@ -64,7 +64,7 @@ struct fsl_ctx {
     double* Uq;         // 5 G, fused path only
     double* faces;      // 10 (NX + NY + NZ), unfused path only
     double* mu;         // G
-    double* sgs;        // scratch for the dynamic model
+    double* sgs;        // scratch for the FP32 dynamic model (mixed contexts)
     double* filt;       // scratch of fsl_filter (in, out and two pass buffers)
     double* u_in;       // HIG
     double* sums;       // partials + 5 results
@@ -102,7 +102,6 @@ void prof_mark(void* arg, int id, int end);
 State5 st5(double* base, long long G);
 CState5 cst5(const double* base, long long G);
 int ensure_sgs(fsl_ctx* c);
-SgsBufs sgs_bufs(fsl_ctx* c);
 int check_launch(fsl_ctx* c, const char* what);
 Bnd bnd_now(fsl_ctx* c);
 int ctx_ghost(fsl_ctx* c, const double* state, bool z = true);

@@ -83,14 +83,17 @@ struct CState5 {
 
 // ---------------------------------------------------------------------
 // Limiter, src/reconstruction.c:17-24 (Chakravarthy-Osher minmod, b = 4)
+// x y <= 0 gives 0; otherwise the reference returns min(x, |4y|) for x > 0
+// and, for x < 0, x when -x < |4y|, else -|4y|: sign(x) min(|x|, |4y|), with
+// a tie choosing the same value.  Written as min and copysign (no compare /
+// select chains); on finite operands the result is the reference's bit for
+// bit.  (With a NaN operand the reference may return NaN where min returns
+// the other operand; a NaN state fails the time-step guard.)
 template <class T>
 __device__ __forceinline__ T minmod(T x, T y)
 {
-    if (x * y <= T(0)) return T(0);
-    y *= T(4.0);
-    T ay = (y > T(0)) ? y : -y;
-    if (x > T(0)) return (x < ay) ? x : ay;
-    return (-x < ay) ? x : -ay;
+    const T m = __builtin_fmin(__builtin_fabs(x), __builtin_fabs(T(4.0) * y));
+    return (x * y <= T(0)) ? T(0) : __builtin_copysign(m, x);
 }
 
 // Per-cell quantities of the characteristic transform,
@@ -103,15 +106,17 @@ struct Prim {
     T vhh[3], vee[3];
 };
 
+// R = 1 / q[0] and vel = q[1..3] R given: the values vel_recip computes for
+// the same cell (the same operations), which a caller may carry over
 template <class T>
-__device__ __forceinline__ void cell_prims(const T q[5], Prim<T>& c)
+__device__ __forceinline__ void cell_prims_rv(const T q[5], T R, const T vel[3], Prim<T>& c)
 {
 #pragma unroll
     for (int r = 0; r < 5; r++) c.q[r] = q[r];
-    c.R = T(1.) / q[0];
-    c.vel[0] = q[1] * c.R;
-    c.vel[1] = q[2] * c.R;
-    c.vel[2] = q[3] * c.R;
+    c.R = R;
+    c.vel[0] = vel[0];
+    c.vel[1] = vel[1];
+    c.vel[2] = vel[2];
     c.P = (q[4] - T(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * c.R) * T(kK_1);
     c.C = sqrt(T(kK) * c.P * c.R);
     T ff = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
@@ -127,6 +132,14 @@ __device__ __forceinline__ void cell_prims(const T q[5], Prim<T>& c)
         c.vee[r] = c.vel[r] * c.ee;
     }
     c.nn = T(-0.5) * c.gg;
+}
+
+template <class T>
+__device__ __forceinline__ void cell_prims(const T q[5], Prim<T>& c)
+{
+    const T R = T(1.) / q[0];
+    const T vel[3] = {q[1] * R, q[2] * R, q[3] * R};
+    cell_prims_rv(q, R, vel, c);
 }
 
 // Reconstruction along one direction (src/reconstruction.c:59-170 for x;
@@ -223,13 +236,17 @@ __device__ __forceinline__ void riemann(const T l[5], const T r[5], T& Ro, T vel
     T Un = T(0.5) * (lv[N] + rv[N]);
     const T Cm = T(0.5) * (lC + rC);
     const T C_p = Un + Cm, C_m = Un - Cm, C_o = Un;
-    T al_p, al_m, al_o, J_p, J_m, J_o, a;
-    if (C_p > T(0.)) { al_p = T(kK_1_2) * (ljm - ljp) / ljo; J_p = ljp; }
-    else             { al_p = T(kK_1_2) * (rjm - rjp) / rjo; J_p = rjp; }
-    if (C_m > T(0.)) { al_m = T(kK_1_2) * (ljp - ljm) / ljo; J_m = ljm; }
-    else             { al_m = T(kK_1_2) * (rjp - rjm) / rjo; J_m = rjm; }
-    if (C_o > T(0.)) { a = T(kH05K) * (ljp - ljm); al_o = -a * a; J_o = ljo; }
-    else             { a = T(kH05K) * (rjp - rjm); al_o = -a * a; J_o = rjo; }
+    // each coefficient from the side its characteristic comes from: the
+    // operands are chosen first and divided once (the same operations on the
+    // same values as the reference's branches, without evaluating both)
+    const bool sp = C_p > T(0.), sm = C_m > T(0.), so = C_o > T(0.);
+    const T al_p = T(kK_1_2) * (sp ? ljm - ljp : rjm - rjp) / (sp ? ljo : rjo);
+    const T J_p = sp ? ljp : rjp;
+    const T al_m = T(kK_1_2) * (sm ? ljp - ljm : rjp - rjm) / (sm ? ljo : rjo);
+    const T J_m = sm ? ljm : rjm;
+    const T a = T(kH05K) * (so ? ljp - ljm : rjp - rjm);
+    const T al_o = -a * a;
+    const T J_o = so ? ljo : rjo;
     const T P = (J_p - J_m) / (al_p - al_m);
     Un = J_p - al_p * P;
     Ro = (J_o - P) / al_o;
@@ -243,10 +260,11 @@ __device__ __forceinline__ void riemann(const T l[5], const T r[5], T& Ro, T vel
 // Cell velocity by reciprocal density, src/fluxes.c:124-127 (and the
 // reconstruction's U = u2 * (1/u1), src/reconstruction.c:71-74).
 template <class T>
-__device__ __forceinline__ void vel_recip(T r, T m0, T m1, T m2, T v[3])
+__device__ __forceinline__ void vel_recip(T r, T m0, T m1, T m2, T v[3], T* rr = nullptr)
 {
     const T RU = T(1.) / r;
     v[0] = m0 * RU; v[1] = m1 * RU; v[2] = m2 * RU;
+    if (rr) *rr = RU;
 }
 
 // Cell velocity by division and temperature, src/fluxes.c:117-122

@@ -19,9 +19,17 @@
 // three-plane LDS ring of 1/rho-velocities feeds the tangential gradients.
 // HBM traffic per cell and stage is the stage state (plus halo re-reads that
 // hit L2), the step state for stages 2 and 3, mu_SGS and the new state.
+//
+// Addressing.  Every access goes through a buffer resource of one x plane
+// (fsl_buffer.hpp): the plane is uniform over the workgroup, so its base is
+// scalar, and a lane adds a 32-bit in-plane byte offset.  The offsets of a
+// lane's own cell and of its y and z neighbours do not change along the
+// march and are computed once, so the 30 state loads of a plane step carry
+// no per-lane address arithmetic.
 #include <stdlib.h>
 
 #include "fsl.h"
+#include "fsl_buffer.hpp"
 #include "fsl_internal.hpp"
 
 namespace fsl {
@@ -50,22 +58,17 @@ struct FusedArgs {
     Bnd b;
     double dtX, dtY, dtZ;
     int segL, nseg, ntj, ntk, nblocks, xcd_remap;
+    unsigned fb;                   // bytes of one field (G doubles) in the one-resource-per-plane form
     unsigned long long* cfl_out;   // stage 3: CFL maximum of the new state (or null)
     unsigned long long* cfl_zero;  // stage 2: CFL cell to clear for the following stage 3 (or null)
     int cfl_mode;
 };
 
-__device__ __forceinline__ void load5(const CState5& s, long long c, double q[5])
-{
-#pragma unroll
-    for (int f = 0; f < 5; f++) q[f] = s.f[f][c];
-}
-
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 
 // BJ x BK threads (BK on the fast lane index), MINW: minimum waves per SIMD
-// the register allocation must allow.
-template <int STAGE, int BJ, int BK, int MINW>
+// the register allocation must allow; PK: one buffer resource per state plane.
+template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
 {
     constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK;
@@ -112,44 +115,68 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
     int zkm = kc - 1, zkp = kc + 1;
     if (kc == 0) { zkm = D - 1; zkp = D + 1; }
     else if (kc == D + 1) { zkm = 0; zkp = 2; }
-    const long long sj = g.sj, si = g.si;
-    const long long col = (long long)jc * sj + kc;   // (j, k) offset of this column
+    const long long si = g.si;
+    const int pb = (int)(si * 8);                 // bytes of one plane of one field
+    const unsigned fb = a.fb;
+    const unsigned sjb = (unsigned)g.sj * 8u;
+    // in-plane byte offsets of this lane's cell and of its neighbours
+    const unsigned oc = (unsigned)jc * sjb + (unsigned)kc * 8u;
+    const unsigned oyp = (unsigned)jp * sjb + (unsigned)kc * 8u, oym = (unsigned)jm * sjb + (unsigned)kc * 8u;
+    const unsigned ozp = (unsigned)jc * sjb + (unsigned)zkp * 8u, ozm = (unsigned)jc * sjb + (unsigned)zkm * 8u;
+    // mu_SGS of the y and z face neighbours (cells (j + 1) and (k + 1), read
+    // by face lanes only, whose j <= HIG and k <= DEP)
+    const unsigned omy = oc + sjb, omz = oc + 8u;
 
     double cfl_m = 0.0;           // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];   // stage state at planes i-1, i, i+1 (own column)
     double xp_prev[5];            // + x state of plane i-1
     double Fx_prev[5];
     double ly[5], lz[5];
+    double mu_x = 0., mu_y = 0., mu_z = 0.;   // mu_SGS of plane i-1: own cell, (j+1), (k+1)
 #pragma unroll
     for (int f = 0; f < 5; f++) { xp_prev[f] = 0.; Fx_prev[f] = 0.; ly[f] = 0.; lz[f] = 0.; }
 
+    // 1/rho and velocity of the next plane's cell (src/reconstruction.c:71-74
+    // and src/fluxes.c:124-127 compute them alike), carried to the plane
+    // where that cell is reconstructed
+    double rr2, vr2[3];
     {
         const int ia = max(i0 - 2, 0), ib = i0 - 1;
-        load5(a.in, (long long)ia * si + col, w1);
-        load5(a.in, (long long)ib * si + col, w2);
-        double vr[3];
-        vel_recip(w2[0], w2[1], w2[2], w2[3], vr);
+        load5<PK>(a.in, (long long)ia * si, pb, fb, oc, w1);
+        load5<PK>(a.in, (long long)ib * si, pb, fb, oc, w2);
+        vel_recip(w2[0], w2[1], w2[2], w2[3], vr2, &rr2);
         const int r = ring(ib);
 #pragma unroll
-        for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr[e];
+        for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr2[e];
     }
 
     for (int i = i0 - 1; i <= i1; i++) {
 #pragma unroll
         for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
-        load5(a.in, (long long)min(i + 1, L + 1) * si + col, w2);
-        // step state of the cell this iteration updates (i - 1), issued early:
-        // its use after the last barrier would otherwise wait for the load
+        const long long pl = (long long)i * si;
+        // loads in the order of first use: this plane's y neighbours (y
+        // reconstruction), the next plane's cell (1/rho velocity), mu_SGS of
+        // this plane, the step state of the cell this iteration updates (i - 1)
+        const bool inseg = (i >= i0 && i < i1);
+        double qyp[5], qym[5];
+        if (inseg) {
+            load5<PK>(a.in, pl, pb, fb, oyp, qyp);
+            load5<PK>(a.in, pl, pb, fb, oym, qym);
+        }
+        load5<PK>(a.in, (long long)min(i + 1, L + 1) * si, pb, fb, oc, w2);
+        double m_x = 0., m_y = 0., m_z = 0.;
+        if (a.ph.dynamic) {
+            const __amdgpu_buffer_rsrc_t mr = plane_rsrc(a.mu, pl, pb);
+            m_x = bload(mr, oc);
+            // also on the plane before the segment: the segment's first y/z faces need it
+            if (yface) m_y = bload(mr, omy);
+            if (zface) m_z = bload(mr, omz);
+        }
         double bq[5];
-        if (STAGE > 1) load5(a.base, (long long)max(i - 1, 0) * si + col, bq);
+        if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, oc, bq);
         lds_barrier();   // the previous plane's LDS readers are done
         const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {
-            double vr[3];
-            vel_recip(w2[0], w2[1], w2[2], w2[3], vr);
-            const int r = ring(i + 1);
-#pragma unroll
-            for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr[e];
             double vd[3], Tc;
             vel_div_T(w1[0], w1[1], w1[2], w1[3], w1[4], vd, Tc);
             s_visc[vc][0][c] = vd[0];
@@ -159,21 +186,23 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
             s_visc[vc][4][c] = w1[0];
         }
         Prim<double> P;
-        cell_prims(w1, P);
-        const bool inseg = (i >= i0 && i < i1);
+        cell_prims_rv(w1, rr2, vr2, P);
         if (inseg) {
-            const long long pl = (long long)i * si;
-            double qp[5], qm[5], pl5[5], mi5[5];
-            load5(a.in, pl + (long long)jp * sj + kc, qp);
-            load5(a.in, pl + (long long)jm * sj + kc, qm);
-            recon_dir<2, 3, 1>(P, qp, qm, false, pl5, mi5);
+            double qzp[5], qzm[5], pl5[5], mi5[5];
+            load5<PK>(a.in, pl, pb, fb, ozp, qzp);
+            load5<PK>(a.in, pl, pb, fb, ozm, qzm);
+            recon_dir<2, 3, 1>(P, qyp, qym, false, pl5, mi5);
 #pragma unroll
             for (int f = 0; f < 5; f++) { s_fy[f][c] = pl5[f]; s_ym[f][c] = mi5[f]; }
-            load5(a.in, pl + (long long)jc * sj + zkp, qp);
-            load5(a.in, pl + (long long)jc * sj + zkm, qm);
-            recon_dir<3, 1, 2>(P, qp, qm, a.ph.z_quirk != 0, pl5, mi5);
+            recon_dir<3, 1, 2>(P, qzp, qzm, a.ph.z_quirk != 0, pl5, mi5);
 #pragma unroll
             for (int f = 0; f < 5; f++) { s_fz[f][c] = pl5[f]; s_zm[f][c] = mi5[f]; }
+        }
+        {
+            vel_recip(w2[0], w2[1], w2[2], w2[3], vr2, &rr2);
+            const int r = ring(i + 1);
+#pragma unroll
+            for (int e = 0; e < 3; e++) s_vr[r][e][c] = vr2[e];
         }
         double xp[5], xm[5];
         recon_dir<1, 2, 3>(P, w2, w0, false, xp, xm);
@@ -210,16 +239,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                 gr[q][2] = (s_vr[rc][q][c + 1] + s_vr[rb][q][c + 1] - s_vr[rc][q][c - 1] -
                             s_vr[rb][q][c - 1]) * a.ph.r4del[2];
             }
-            double mu_dyn = 0.;
-            if (a.ph.dynamic) {
-                const long long m = (long long)i * si + col;
-                mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
-            }
+            const double mu_dyn = a.ph.dynamic ? 0.5 * (m_x + mu_x) : 0.;
             double Fx[5];
             face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
                          s_visc[vb][4][c], gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
-                const long long cc = (long long)(i - 1) * si + col;
+                const long long po = (long long)(i - 1) * si;
                 double ov[5];
 #pragma unroll
                 for (int f = 0; f < 5; f++) {
@@ -229,7 +254,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                     if (STAGE == 1) o = s;
                     else if (STAGE == 2) o = 0.75 * bq[f] + 0.25 * s;
                     else o = (1. / 3.) * bq[f] + (2. / 3.) * s;
-                    a.out.f[f][cc] = o;
+                    bstore(field_rsrc<PK>(a.out, f, po, pb, fb), oc, PK ? f * fb : 0u, o);
                     ov[f] = o;
                 }
                 if (STAGE == 3 && a.cfl_out) {
@@ -278,11 +303,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                     gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
                                 s_vr[rc][q][c - 1]) * a.ph.r4del[2];
                 }
-                double mu_dyn = 0.;
-                if (a.ph.dynamic) {
-                    const long long m = (long long)i * si + (long long)(jc + 1) * sj + kc;
-                    mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
-                }
+                // cell (i, j + 1) and its x neighbour (src/fluxes.c:360)
+                const double mu_dyn = a.ph.dynamic ? 0.5 * (m_y + mu_y) : 0.;
                 double Fy[5];
                 face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + BK], Tb, s_visc[vc][4][c + BK], Rb,
                              gr, mu_dyn, Fy);
@@ -305,11 +327,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                     gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
                                 s_vr[rc][q][c - BK]) * a.ph.r4del[1];
                 }
-                double mu_dyn = 0.;
-                if (a.ph.dynamic) {
-                    const long long m = (long long)i * si + (long long)jc * sj + kc + 1;
-                    mu_dyn = 0.5 * (a.mu[m] + a.mu[m - si]);
-                }
+                // cell (i, k + 1) and its x neighbour (src/fluxes.c:531)
+                const double mu_dyn = a.ph.dynamic ? 0.5 * (m_z + mu_z) : 0.;
                 double Fz[5];
                 face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + 1], Tb, s_visc[vc][4][c + 1], Rb,
                              gr, mu_dyn, Fz);
@@ -317,6 +336,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
         }
+        mu_x = m_x;
+        mu_y = m_y;
+        mu_z = m_z;
         lds_barrier();
         if (inseg && interior) {
 #pragma unroll
@@ -394,7 +416,7 @@ Launch plan(const Geo& g, int BJ, int BK, int per_cu)
     return p;
 }
 
-template <int BJ, int BK, int MINW>
+template <int BJ, int BK, int MINW, bool PK>
 void launch_variant(hipStream_t s, int stage, FusedArgs a, int per_cu)
 {
     const Launch p = plan(a.g, BJ, BK, per_cu);
@@ -404,20 +426,20 @@ void launch_variant(hipStream_t s, int stage, FusedArgs a, int per_cu)
     a.ntk = p.ntk;
     a.nblocks = p.nblocks;
     a.xcd_remap = p.remap;
-    if (stage == 1) hipLaunchKernelGGL((k_fused<1, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
-    else if (stage == 2) hipLaunchKernelGGL((k_fused<2, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
-    else hipLaunchKernelGGL((k_fused<3, BJ, BK, MINW>), p.grid, p.block, 0, s, a);
+    if (stage == 1) hipLaunchKernelGGL((k_fused<1, BJ, BK, MINW, PK>), p.grid, p.block, 0, s, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_fused<2, BJ, BK, MINW, PK>), p.grid, p.block, 0, s, a);
+    else hipLaunchKernelGGL((k_fused<3, BJ, BK, MINW, PK>), p.grid, p.block, 0, s, a);
 }
 
-// Tile shape / occupancy variants: 0 = 16x16, 2 waves/SIMD (default);
-// 1 = 16x16, 1 wave/SIMD; 2 = 16x32, 2 waves/SIMD; 3 = 16x32, 1 wave/SIMD
+// Occupancy variants of the 16 x 16 tile: 0 = 2 waves/SIMD (default),
+// 1 = 1 wave/SIMD (no register limit below 512)
 int fused_variant()
 {
     static int v = -1;
     if (v < 0) {
         const char* e = getenv("FSL_FUSED_VARIANT");
         v = e ? atoi(e) : 0;
-        if (v < 0 || v > 3) v = 0;
+        if (v < 0 || v > 1) v = 0;
     }
     return v;
 }
@@ -466,11 +488,21 @@ int fused_stage(fsl_ctx* ctx, int stage)
     ctx_prof(ctx, K_FUSED_XYZ, 0);
     if (v.precision == FSL_PREC_MIXED) {
         launch_fused_mixed(v, stage, in, out);
-    } else switch (fused_variant()) {
-    case 1: launch_variant<16, 16, 1>(v.stream, stage, a, 1); break;
-    case 2: launch_variant<16, 32, 2>(v.stream, stage, a, 1); break;
-    case 3: launch_variant<16, 32, 1>(v.stream, stage, a, 1); break;
-    default: launch_variant<16, 16, 2>(v.stream, stage, a, 2); break;
+    } else {
+        // one buffer resource per state plane where the five fields fit 32-bit
+        // offsets, (4 G + one plane) * 8 < 2^32 bytes; otherwise one per field
+        const unsigned long long span = (4ull * (unsigned long long)v.g.G + (unsigned long long)v.g.si) * 8ull;
+        const char* e = getenv("FSL_FP64_FIELD_RSRC");
+        const bool pk = span <= 0xffffffffull && !(e && atoi(e) == 1);
+        a.fb = (unsigned)(pk ? v.g.G * 8 : 0);
+        if (pk) {
+            switch (fused_variant()) {
+            case 1: launch_variant<16, 16, 1, true>(v.stream, stage, a, 1); break;
+            default: launch_variant<16, 16, 2, true>(v.stream, stage, a, 2); break;
+            }
+        } else {
+            launch_variant<16, 16, 2, false>(v.stream, stage, a, 2);
+        }
     }
     ctx_prof(ctx, K_FUSED_XYZ, 1);
     rc = ctx_check(ctx, "fused stage");

@@ -66,20 +66,15 @@ int sums_nblocks(const Geo& g);
 void launch_copy(hipStream_t s, const double* a, double* b, size_t n);
 
 // --- dynamic Smagorinsky (fsl_sgs.hip) ---
-// Intermediates of the tiled model: filtered velocities (zero outside the
-// interior and the i = 0 plane), Leonard stress (6 distinct components),
-// A = |S| S (6, ghost layer kept 0) and |S|.
-struct SgsBufs {
-    double* uf[3];
-    double* Lij[6];     // 11, 12, 13, 22, 23, 33
-    double* Aij[6];
-    double* mag;
-};
-size_t sgs_scratch_arrays();   // number of cell arrays SgsBufs needs
-void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
+// FP64: the reference's operations in one x-marching kernel without
+// intermediate arrays (fsl_sgs_march.hip)
+void launch_sgs(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph,
                 void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
-// the same model in FP32 (filters, Germano contraction, scratch) on the FP64
-// scratch allocation; mu_SGS stays an FP64 array
+// bytes of scratch the FP32 model needs (16 float cell arrays)
+size_t sgs_f32_scratch_bytes(const Geo& g);
+void launch_sgs_march(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, float* cd_out);
+// the same model in FP32 (filters, Germano contraction) with two tiled
+// kernels and 16 float scratch arrays; mu_SGS stays an FP64 array
 void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
                     void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
 

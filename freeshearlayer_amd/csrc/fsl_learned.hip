@@ -133,7 +133,7 @@ int dynamic_mu(fsl_ctx* c, float* cd)
     if (c->cfg.precision == FSL_PREC_MIXED)
         launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, cd);
     else
-        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, sgs_bufs(c), c->g, c->ph, prof_mark, c, cd);
+        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, c->g, c->ph, prof_mark, c, cd);
     return check_launch(c, "dynamic_smagorinsky");
 }
 

@@ -15,8 +15,10 @@
 //   x pass  out on i in [1,L], j in [0,H], k in [0,D]; elsewhere out = t1
 // with each sum accumulated as ((0 + a h) + b h) + c h.
 //
-// Two tiled kernels do the whole model.  Each workgroup owns a 2 x 8 x 32
-// tile of interior cells and stages a one-cell-halo box in LDS:
+// The FP64 model (the reference's operations, bit-identical) is one x-marching
+// kernel, fsl_sgs_march.hip.  The FP32 model of mixed-precision contexts is
+// the two tiled kernels here.  Each workgroup owns a 2 x 8 x 32 tile of
+// interior cells and stages a one-cell-halo box in LDS:
 //   k_sgs_a  U -> u, v, w on the box; per field (u, v, w and the six
 //            products) the y, z and x passes on the box; Leonard stress,
 //            |S| and A = |S| S at the tile's cells; writes u^, L, A, |S|
@@ -476,8 +478,6 @@ void run_filters(hipStream_t s, int nf, const double* const* in, double* const* 
 
 }  // namespace
 
-size_t sgs_scratch_arrays() { return 3 + 6 + 6 + 1; }
-
 void launch_filter(hipStream_t s, const double* in, double* out, double* t1, double* t2, const Geo& g,
                    const double h[3], void (*prof)(void*, int, int), void* pa)
 {
@@ -519,16 +519,20 @@ void sgs_pair(hipStream_t s, CState5 U, double* mu, R* const uf[3], R* const Lij
 
 }  // namespace
 
-void launch_sgs(hipStream_t s, CState5 U, double* mu, const SgsBufs& B, const Geo& g, const Phys& ph,
-                void (*prof)(void*, int, int), void* pa, float* cd_out)
+void launch_sgs(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, void (*prof)(void*, int, int),
+                void* pa, float* cd_out)
 {
-    sgs_pair<double>(s, U, mu, B.uf, B.Lij, B.Aij, B.mag, g, ph, prof, pa, cd_out);
+    if (prof) prof(pa, K_SGS_FINAL, 0);
+    launch_sgs_march(s, U, mu, g, ph, cd_out);
+    if (prof) prof(pa, K_SGS_FINAL, 1);
 }
+
+size_t sgs_f32_scratch_bytes(const Geo& g) { return (size_t)(3 + 6 + 6 + 1) * (size_t)g.G * sizeof(float); }
 
 void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
                     void (*prof)(void*, int, int), void* pa, float* cd_out)
 {
-    // the FP64 scratch holds 16 cell arrays; the FP32 form uses 16 float arrays in its first half
+    // 16 float cell arrays: u^ (3), Leonard stress (6), A (6), |S|
     float* p = (float*)scratch;
     float *uf[3], *Lij[6], *Aij[6];
     for (int r = 0; r < 3; r++) { uf[r] = p; p += g.G; }
