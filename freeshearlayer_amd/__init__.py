@@ -63,7 +63,8 @@ class FslLearnedStatus(C.Structure):
     _fields_ = [("interval", C.c_int), ("tol", C.c_double), ("in_fallback", C.c_int),
                 ("last_error", C.c_double), ("judged", C.c_longlong), ("fallbacks", C.c_longlong),
                 ("learned_steps", C.c_longlong), ("dynamic_steps", C.c_longlong), ("scale", C.c_double),
-                ("adapt", C.c_int)]
+                ("adapt", C.c_int), ("profile_tol", C.c_double), ("local_tol", C.c_double),
+                ("last_profile_error", C.c_double), ("last_local_error", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -182,6 +183,9 @@ def load():
         "fsl_learned_history": (i_, [vp, i_, _dp, _dp]),
         "fsl_learned_get_status": (i_, [vp, C.POINTER(FslLearnedStatus)]),
         "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
+        "fsl_learned_judge_errors": (i_, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "fsl_learned_set_tolerances": (i_, [vp, d_, d_]),
+        "fsl_learned_history_errors": (i_, [vp, i_, _dp, _dp, _dp, _dp]),
         "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
         "fsl_output_fields": (i_, [vp, i_, i_, _dp]),
         "fsl_conservation_sums_noalloc": (i_, [vp, u_, u_, u_, vp, vp]),
@@ -508,43 +512,31 @@ class Context:
     def learned_set_scale(self, scale=1.0, adapt=True):
         self._chk(self.lib.fsl_learned_set_scale(self.h, scale, int(adapt)), "learned_set_scale")
 
-    def learned_history(self):
+    def learned_history(self, errors=False):
+        """Judge history: (errors, scales), or with errors=True (errors,
+        profile errors, local errors, scales)."""
         n = self.lib.fsl_learned_history(self.h, 0, None, None)
-        e = np.zeros(max(n, 1))
-        sc = np.zeros(max(n, 1))
-        self.lib.fsl_learned_history(self.h, n, _ptr(e), _ptr(sc))
-        return e[:n], sc[:n]
+        e, pe, le, sc = (np.zeros(max(n, 1)) for _ in range(4))
+        self.lib.fsl_learned_history_errors(self.h, n, _ptr(e), _ptr(pe), _ptr(le), _ptr(sc))
+        return (e[:n], pe[:n], le[:n], sc[:n]) if errors else (e[:n], sc[:n])
+
+    def learned_set_tolerances(self, profile_tol, local_tol):
+        self._chk(self.lib.fsl_learned_set_tolerances(self.h, profile_tol, local_tol), "learned_set_tolerances")
 
     def learned_status(self):
         s = FslLearnedStatus()
         self._chk(self.lib.fsl_learned_get_status(self.h, C.byref(s)), "learned_status")
         return s.as_dict()
 
-    def learned_judge(self):
-        e, dd, dl = C.c_double(), C.c_double(), C.c_double()
-        self._chk(self.lib.fsl_learned_judge(self.h, C.byref(e), C.byref(dd), C.byref(dl)), "learned_judge")
+    def learned_judge(self, errors=False):
+        """One judge on the step state: (error, D_dynamic, D_learned), or with
+        errors=True (error, profile error, local error, D_dynamic, D_learned)."""
+        e, pe, le, dd, dl = (C.c_double() for _ in range(5))
+        self._chk(self.lib.fsl_learned_judge_errors(self.h, C.byref(e), C.byref(pe), C.byref(le), C.byref(dd),
+                                                    C.byref(dl)), "learned_judge")
+        if errors:
+            return e.value, pe.value, le.value, dd.value, dl.value
         return e.value, dd.value, dl.value
-
-    def output_fields(self, k0=1, nk=None):
-        """Output()'s nine computed columns (rho, u, v, w, p, T, |omega|, Q,
-        mu_T/mu_L) of planes k0 .. k0+nk-1: array (nk, HIG, LEN, 9)."""
-        case = self.case
-        if nk is None:
-            nk = case.DEP - k0 + 1
-        out = np.zeros((nk, case.HIG, case.LEN, 9))
-        self._chk(self.lib.fsl_output_fields(self.h, k0, nk, _ptr(out)), "output_fields")
-        return out
-
-    def write_plt(self, path=None):
-        self._chk(self.lib.fsl_write_plt(self.h, path.encode() if path else None), "write_plt")
-
-    def probe_values(self):
-        out = np.zeros((6, self.case.HIG))
-        self._chk(self.lib.fsl_probe_values(self.h, _ptr(out)), "probe_values")
-        return out
-
-    def write_probes(self, path=None, real_bytes=8):
-        self._chk(self.lib.fsl_write_probes(self.h, path.encode() if path else None, real_bytes), "write_probes")
 
     def mfma_probe(self, A, B):
         A = _f32(A, (16, 16))

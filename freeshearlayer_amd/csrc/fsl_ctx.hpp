@@ -38,7 +38,7 @@ struct LearnedState {
     float* dw = nullptr;       // delta_w(x), L + 2
     int dw_valid = 0;
     double* mu_alt = nullptr;  // second mu_SGS array for the judge
-    double* part = nullptr;    // dissipation partials (2 x blocks)
+    double* part = nullptr;    // dissipation partials per x band and row (5 sums)
     double* hpart = nullptr;   // pinned host copy of the partials
     int interval = 50;         // judge every interval steps (0: never)
     double tol = 0.10;         // relative dissipation error that triggers the fallback
@@ -46,7 +46,11 @@ struct LearnedState {
     int adapt = 1;             // judge rescales the network's Cd to the dynamic model's dissipation
     float scale = 1.f;         // current output scale of the network's Cd
     double last_error = -1.;
-    std::vector<float> hist_err, hist_scale;   // per judge: error, scale in use
+    double profile_tol = 1e30;       // bounds on the judge's profile and local errors (judge_error)
+    double local_tol = 1e30;
+    double last_profile_error = -1.;
+    double last_local_error = -1.;
+    std::vector<float> hist_err, hist_perr, hist_lerr, hist_scale;   // per judge: errors, scale in use
     long long judged = 0, fallbacks = 0, learned_steps = 0, dynamic_steps = 0;
 };
 

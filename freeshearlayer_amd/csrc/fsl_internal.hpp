@@ -99,9 +99,14 @@ void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long l
                            float winv);
 void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t);
 void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D);
-int dissipation_nblocks();
+// per-row (y) sums over x band s of the SGS dissipation densities e_a, e_b
+// of two mu_SGS fields and of e_a^2, e_b^2, e_a e_b:
+// rows[(q NS + s) HIG + j - 1], q = 0..4, s = 0 .. NS - 1 with
+// NS = dissipation_splits(g) = min(kDissBands, LEN) x bands
+constexpr int kDissBands = 16;
+int dissipation_splits(const Geo& g);
 void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu_a,
-                            const double* mu_b, double* partial_a, double* partial_b);
+                            const double* mu_b, double* rows);
 void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out);   // 7 L H doubles
 void launch_diag_dw(hipStream_t s, const double* zs, const Geo& g, double dU, double dy, float fallback,
                     float* dw);

@@ -74,10 +74,10 @@ int ensure_judge(fsl_ctx* c)
     HIPCHK(c, hipMalloc(&L.mu_alt, c->g.G * sizeof(double)));
     // the ghost layer of mu_SGS reads as 0 (src/turbulence.c:436-445 writes the interior only)
     HIPCHK(c, hipMemsetAsync(L.mu_alt, 0, c->g.G * sizeof(double), c->stream));
-    const int nb = dissipation_nblocks();
-    HIPCHK(c, hipMalloc(&L.part, 2 * nb * sizeof(double)));
-    HIPCHK(c, hipHostMalloc((void**)&L.hpart, 2 * nb * sizeof(double), hipHostMallocDefault));
-    c->bytes += c->g.G * sizeof(double) + 2 * nb * sizeof(double);
+    const size_t nr = (size_t)dissipation_splits(c->g) * c->g.H;
+    HIPCHK(c, hipMalloc(&L.part, 5 * nr * sizeof(double)));
+    HIPCHK(c, hipHostMalloc((void**)&L.hpart, 5 * nr * sizeof(double), hipHostMallocDefault));
+    c->bytes += c->g.G * sizeof(double) + 5 * nr * sizeof(double);
     return FSL_OK;
 }
 
@@ -137,25 +137,56 @@ int dynamic_mu(fsl_ctx* c, float* cd)
     return check_launch(c, "dynamic_smagorinsky");
 }
 
-// relative difference of the SGS dissipation of mu_a (reference) and mu_b
-int judge_error(fsl_ctx* c, const double* mu_a, const double* mu_b, double* err, double* da, double* db)
+// The judge's measures of mu_b (the network's) against mu_a (the dynamic
+// model's), with e = mu |S|^2 the SGS dissipation density and D its sum over
+// the interior:
+//   err  = |D_b - D_a| / D_a                              (global level)
+//   perr = || s P_b - P_a ||_2 / || P_a ||_2, P(j) the x-z sum of e in row j
+//   lerr = || s e_b - e_a ||_2 / || e_a ||_2 over the cells
+// with s = D_a / D_b, the level the judge's rescale sets: perr measures the
+// distribution of the dissipation across the layer, lerr its cell-by-cell
+// distribution.
+struct JudgeErr {
+    double err, perr, lerr, da, db;
+};
+
+int judge_error(fsl_ctx* c, const double* mu_a, const double* mu_b, JudgeErr* out)
 {
     LearnedState& L = c->lrn;
-    const int nb = dissipation_nblocks();
-    launch_sgs_dissipation(c->stream, cst5(c->U, c->g.G), c->g, c->ph, mu_a, mu_b, L.part, L.part + nb);
+    const int ns = dissipation_splits(c->g), H = c->g.H;
+    const size_t nr = (size_t)ns * H;
+    launch_sgs_dissipation(c->stream, cst5(c->U, c->g.G), c->g, c->ph, mu_a, mu_b, L.part);
     int rc = check_launch(c, "dissipation");
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(L.hpart, L.part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(L.hpart, L.part, 5 * nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double sa = 0, sb = 0;
-    for (int b = 0; b < nb; b++) {
-        sa += L.hpart[b];
-        sb += L.hpart[nb + b];
+    // rows[q][j]: the NS partials of each row added in order
+    std::vector<double> rows(5 * (size_t)H, 0.);
+    for (int q = 0; q < 5; q++)
+        for (int p = 0; p < ns; p++)
+            for (int j = 0; j < H; j++) rows[(size_t)q * H + j] += L.hpart[((size_t)q * ns + p) * H + j];
+    double tot[5] = {0., 0., 0., 0., 0.};
+    for (int q = 0; q < 5; q++)
+        for (int j = 0; j < H; j++) tot[q] += rows[(size_t)q * H + j];
+    const double sa = tot[0], sb = tot[1];
+    out->da = sa;
+    out->db = sb;
+    if (sa > 0) out->err = fabs(sb - sa) / sa;
+    else out->err = sb > 0 ? INFINITY : 0.;
+    if (sa > 0 && sb > 0) {
+        const double sc = sa / sb;
+        double num = 0., den = 0.;
+        for (int j = 0; j < H; j++) {
+            const double d = sc * rows[(size_t)H + j] - rows[j];
+            num += d * d;
+            den += rows[j] * rows[j];
+        }
+        out->perr = sqrt(num / den);
+        const double l2 = sc * sc * tot[3] - 2. * sc * tot[4] + tot[2];
+        out->lerr = sqrt(fmax(l2, 0.) / tot[2]);
+    } else {
+        out->perr = out->lerr = (sa > 0 || sb > 0) ? INFINITY : 0.;
     }
-    *da = sa;
-    *db = sb;
-    if (sa > 0) *err = fabs(sb - sa) / sa;
-    else *err = sb > 0 ? INFINITY : 0.;
     return FSL_OK;
 }
 
@@ -213,17 +244,22 @@ int learned_sgs_step(fsl_ctx* c)
         if ((rc = ensure_judge(c)) || (rc = update_thickness(c))) return rc;
         if ((rc = dynamic_mu(c, nullptr))) return rc;           // into c->mu
         if ((rc = learned_mu(c, L.mu_alt))) return rc;
-        double err, dd, dl;
-        if ((rc = judge_error(c, c->mu, L.mu_alt, &err, &dd, &dl))) return rc;
+        JudgeErr je;
+        if ((rc = judge_error(c, c->mu, L.mu_alt, &je))) return rc;
+        const double err = je.err, perr = je.perr, dd = je.da, dl = je.db;
         L.judged++;
         L.last_error = err;
+        L.last_profile_error = perr;
+        L.last_local_error = je.lerr;
         L.hist_err.push_back((float)err);
+        L.hist_perr.push_back((float)perr);
+        L.hist_lerr.push_back((float)je.lerr);
         L.hist_scale.push_back(L.scale);
         // global level of the network's coefficient from the dynamic model
         // (the network supplies the spatial distribution): used from the next step
         const float next = (L.adapt && dl > 0 && dd > 0) ? std::min(10.f, std::max(0.1f, (float)(L.scale * dd / dl)))
                                                          : L.scale;
-        if (err > L.tol) {
+        if (err > L.tol || perr > L.profile_tol || je.lerr > L.local_tol) {
             L.fallbacks++;
             L.in_fallback = 1;
             L.dynamic_steps++;
@@ -580,6 +616,15 @@ int fsl_learned_configure(fsl_ctx* c, int interval, double tol)
     return FSL_OK;
 }
 
+int fsl_learned_set_tolerances(fsl_ctx* c, double profile_tol, double local_tol)
+{
+    if (!(profile_tol >= 0) || !(local_tol >= 0))
+        return set_err(c, FSL_ERR_ARG, "tolerances %g %g", profile_tol, local_tol);
+    c->lrn.profile_tol = profile_tol;
+    c->lrn.local_tol = local_tol;
+    return FSL_OK;
+}
+
 int fsl_learned_set_scale(fsl_ctx* c, double scale, int adapt)
 {
     if (!(scale >= 0.1 && scale <= 10.)) return set_err(c, FSL_ERR_ARG, "scale %g outside [0.1, 10]", scale);
@@ -590,10 +635,18 @@ int fsl_learned_set_scale(fsl_ctx* c, double scale, int adapt)
 
 int fsl_learned_history(const fsl_ctx* c, int nmax, double* err, double* scale)
 {
+    return fsl_learned_history_errors(c, nmax, err, nullptr, nullptr, scale);
+}
+
+int fsl_learned_history_errors(const fsl_ctx* c, int nmax, double* err, double* profile_err, double* local_err,
+                               double* scale)
+{
     const LearnedState& L = c->lrn;
     const int n = (int)L.hist_err.size();
     for (int q = 0; q < n && q < nmax; q++) {
         if (err) err[q] = L.hist_err[q];
+        if (profile_err) profile_err[q] = L.hist_perr[q];
+        if (local_err) local_err[q] = L.hist_lerr[q];
         if (scale) scale[q] = L.hist_scale[q];
     }
     return n;
@@ -608,6 +661,10 @@ int fsl_learned_get_status(const fsl_ctx* c, fsl_learned_status* s)
     s->tol = L.tol;
     s->in_fallback = L.in_fallback;
     s->last_error = L.last_error;
+    s->profile_tol = L.profile_tol;
+    s->local_tol = L.local_tol;
+    s->last_profile_error = L.last_profile_error;
+    s->last_local_error = L.last_local_error;
     s->judged = L.judged;
     s->fallbacks = L.fallbacks;
     s->learned_steps = L.learned_steps;
@@ -615,18 +672,26 @@ int fsl_learned_get_status(const fsl_ctx* c, fsl_learned_status* s)
     return FSL_OK;
 }
 
-int fsl_learned_judge(fsl_ctx* c, double* err, double* d_dynamic, double* d_learned)
+int fsl_learned_judge_errors(fsl_ctx* c, double* err, double* profile_err, double* local_err, double* d_dynamic,
+                             double* d_learned)
 {
     int rc = need_ready(c);
     if (rc) return rc;
     if ((rc = ensure_judge(c)) || (rc = update_thickness(c))) return rc;
     if ((rc = dynamic_mu(c, nullptr)) || (rc = learned_mu(c, c->lrn.mu_alt))) return rc;
-    double e, dd, dl;
-    if ((rc = judge_error(c, c->mu, c->lrn.mu_alt, &e, &dd, &dl))) return rc;
-    if (err) *err = e;
-    if (d_dynamic) *d_dynamic = dd;
-    if (d_learned) *d_learned = dl;
+    JudgeErr je;
+    if ((rc = judge_error(c, c->mu, c->lrn.mu_alt, &je))) return rc;
+    if (err) *err = je.err;
+    if (profile_err) *profile_err = je.perr;
+    if (local_err) *local_err = je.lerr;
+    if (d_dynamic) *d_dynamic = je.da;
+    if (d_learned) *d_learned = je.db;
     return FSL_OK;
+}
+
+int fsl_learned_judge(fsl_ctx* c, double* err, double* d_dynamic, double* d_learned)
+{
+    return fsl_learned_judge_errors(c, err, nullptr, nullptr, d_dynamic, d_learned);
 }
 
 int fsl_mfma_probe(fsl_ctx* c, const float* A, const float* B, float* D)

@@ -254,9 +254,14 @@ __device__ __forceinline__ unsigned div_by(const Div32& q, unsigned n)
 
 struct CellDecode {
     Div32 byD, byH;
+    Div32 byHb[2];   // row counts H/8 and H/8 + 1 of the y bands of k_mlp_mu_tr
 };
 
-static CellDecode cell_decode(const Geo& g) { return CellDecode{div32((unsigned)g.D), div32((unsigned)g.H)}; }
+static CellDecode cell_decode(const Geo& g)
+{
+    const unsigned hb = (unsigned)g.H / 8;
+    return CellDecode{div32((unsigned)g.D), div32((unsigned)g.H), {div32(hb ? hb : 1), div32(hb + 1)}};
+}
 
 struct FeatArgs {
     CState5 U;
@@ -491,23 +496,31 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         nm[q] = w.norm[q];
         ns[q] = w.norm[NIN + q];
     }
-    const long long N = (long long)a.g.L * a.g.H * a.g.D;
-    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
     unsigned short* X = sX[wv];
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs
-    // (workgroup b on XCD b % 8), so XCD x walks its own contiguous range of
-    // cell blocks in order and the j +- 1 and i +- 1 rows of the stencil come
-    // from its own L2.  The grid is a multiple of 8.
-    const long long per = gridDim.x >> 3, nb8 = (nblk + 7) >> 3;
-    const long long first = (long long)(blockIdx.x & 7) * nb8;
-    for (long long t = blockIdx.x >> 3; t < nb8; t += per) {
-        const long long blk = first + t;
-        if (blk >= nblk) break;   // uniform over the workgroup
+    // (workgroup b on XCD b % 8), so XCD x takes the y band of rows
+    // [x H / 8, (x + 1) H / 8) and walks its cells plane by plane (k fastest,
+    // then j in the band, then i): the three planes of the stencil's band
+    // stay in that XCD's L2.  The grid is a multiple of 8.
+    const int band = blockIdx.x & 7;
+    const int jlo = band * a.g.H / 8, Hb = (band + 1) * a.g.H / 8 - jlo;
+    const Div32 byHb = a.dec.byHb[Hb == a.g.H / 8 ? 0 : 1];
+    const long long N = (long long)a.g.L * Hb * a.g.D;
+    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
+    const long long per = gridDim.x >> 3;
+    // band-local cell index -> (i, j, k), 1-based
+    auto decode = [&](unsigned t, int& ci, int& cj, int& ck) {
+        const unsigned q = div_by(a.dec.byD, t), r = div_by(byHb, q);
+        ck = (int)(t - q * (unsigned)a.g.D) + 1;
+        cj = jlo + (int)(q - r * (unsigned)Hb) + 1;
+        ci = (int)r + 1;
+    };
+    for (long long blk = blockIdx.x >> 3; blk < nblk; blk += per) {   // uniform over the workgroup
         const long long cell = blk * MU_CELLS + threadIdx.x;
         const bool valid = cell < N;
         // the block's window: from the plane before its first cell's
         int ib, jb, kb;
-        decode_cell(a.g, a.dec, (unsigned)(blk * MU_CELLS), ib, jb, kb);
+        decode((unsigned)(blk * MU_CELLS), ib, jb, kb);
         const long long base = (long long)(ib - 1) * a.g.si;
         const long long wbytes = (a.g.G - base) * 8;
         const int nrec = (int)(wbytes < 0x7fffffffLL ? wbytes : 0x7fffffffLL);
@@ -522,7 +535,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         float f[NIN], smag = 0.f;
         int i = 1, j = 1, k = 1;
         if (valid) {
-            decode_cell(a.g, a.dec, (unsigned)cell, i, j, k);
+            decode((unsigned)cell, i, j, k);
             q.oc = ((unsigned)(i - ib + 1) * (unsigned)a.g.si + (unsigned)j * (unsigned)a.g.sj + (unsigned)k) * 8u;
             smag = cell_features(q, a, i, j, f);
         } else {
@@ -856,39 +869,45 @@ struct SumArgs {
     CellDecode dec;
 };
 
-// sums over interior cells of mu_a |S|^2 and mu_b |S|^2 (the judge's SGS
-// dissipation of two mu_SGS fields, one pass), per block
-__global__ void __launch_bounds__(256) k_sgs_dissipation(SumArgs a, double* partial_a, double* partial_b)
+// SGS dissipation densities e = mu |S|^2 (|S| = sqrt(2 S:S) of the
+// central-difference strain, FP32; the sums FP64) of two mu_SGS fields a, b,
+// summed over z and over the planes of one x band for every y row j: sums
+// e_a, e_b, e_a^2, e_b^2, e_a e_b.  Block (j, s) sums the planes
+// i in [s L / NS, (s + 1) L / NS) + 1 of row j into rows[(q NS + s) H + j - 1],
+// q = 0..4; the host adds the NS bands of each row in order (deterministic).
+__global__ void __launch_bounds__(256) k_sgs_dissipation_rows(SumArgs a, double* rows)
 {
-    __shared__ double red[2][256];
-    const long long N = (long long)a.g.L * a.g.H * a.g.D;
-    double sa = 0., sb = 0.;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        decode_cell(a.g, a.dec, (unsigned)t, i, j, k);
+    __shared__ double red[5][256];
+    const int j = blockIdx.x + 1, s = blockIdx.y, ns = gridDim.y;
+    const int i0 = (int)((long long)s * a.g.L / ns), i1 = (int)((long long)(s + 1) * a.g.L / ns);
+    const long long n = (long long)(i1 - i0) * a.g.D;   // cells of this block
+    double acc[5] = {0., 0., 0., 0., 0.};
+    for (long long t = threadIdx.x; t < n; t += blockDim.x) {
+        const int k = (int)(t % a.g.D) + 1;
+        const int i = i0 + 1 + (int)(t / a.g.D);
         float gr[3][3], lap[3];
         velocity_gradient(a.U, a.g, a.ph, i, j, k, gr, lap);
         const float sm = __builtin_amdgcn_sqrtf(2.f * strain_sq(gr));
         const long long c = cidx(a.g, i, j, k);
-        sa += a.mu_a[c] * (double)sm * (double)sm;
-        sb += a.mu_b[c] * (double)sm * (double)sm;
+        const double ea = a.mu_a[c] * (double)sm * (double)sm, eb = a.mu_b[c] * (double)sm * (double)sm;
+        acc[0] += ea;
+        acc[1] += eb;
+        acc[2] += ea * ea;
+        acc[3] += eb * eb;
+        acc[4] += ea * eb;
     }
-    red[0][threadIdx.x] = sa;
-    red[1][threadIdx.x] = sb;
+#pragma unroll
+    for (int q = 0; q < 5; q++) red[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+#pragma unroll
+            for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + o];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        partial_a[blockIdx.x] = red[0][0];
-        partial_b[blockIdx.x] = red[1][0];
-    }
+    if (threadIdx.x < 5) rows[((long long)threadIdx.x * ns + s) * a.g.H + j - 1] = red[threadIdx.x][0];
 }
-
 
 // z sums of rho, rho u, rho v, rho w, (rho u)^2 / rho, (rho v)^2 / rho,
 // (rho w)^2 / rho for every interior (i, j): one wavefront per (i, j) row,
@@ -1067,10 +1086,13 @@ void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D)
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D);
 }
 
-int dissipation_nblocks() { return 1024; }
+int dissipation_splits(const Geo& g)
+{
+    return g.L < kDissBands ? g.L : kDissBands;
+}
 
 void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, const double* mu_a,
-                            const double* mu_b, double* partial_a, double* partial_b)
+                            const double* mu_b, double* rows)
 {
     SumArgs a;
     a.U = U;
@@ -1079,7 +1101,7 @@ void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& 
     a.mu_a = mu_a;
     a.mu_b = mu_b;
     a.dec = cell_decode(g);
-    hipLaunchKernelGGL(k_sgs_dissipation, dim3(1024), dim3(256), 0, s, a, partial_a, partial_b);
+    hipLaunchKernelGGL(k_sgs_dissipation_rows, dim3(g.H, dissipation_splits(g)), dim3(256), 0, s, a, rows);
 }
 
 void launch_diag_zsum(hipStream_t s, CState5 U, const Geo& g, double* out)

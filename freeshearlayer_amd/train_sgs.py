@@ -39,20 +39,39 @@ def log(*a):
     print(*a, flush=True)
 
 
-def generate(mesh, mach, reynolds, warmup, steps, every, stride, seed):
+def generate(mesh, mach, reynolds, pert, warmup, steps, every, stride, seed, early=0, early_stride=None,
+             early_every=1, ic_stride=None):
+    """One FP64 dynamic-model run; samples every `every` steps after `warmup`,
+    and, with early > 0, also of the initial state and of every
+    `early_every`-th of the first `early` steps (every `early_stride`-th
+    cell): the laminar-plus-noise states and the transition out of them
+    that the closure meets at its first judges; `ic_stride` (default
+    `early_stride`) is the cell stride on the initial state itself."""
     L, H, D = mesh
     ctx, case, U, u_in, _ = shear_case.make(L, H, D, sgs="dynamic", precision="fp64", mach=mach,
-                                            reynolds=reynolds, seed=seed)
+                                            reynolds=reynolds, pert=pert, seed=seed)
     N = L * H * D
     nsnap = steps // every
     per = (N + stride - 1) // stride
-    ctx.mlp_reserve(nsnap * per)
+    est = early_stride or stride
+    ist = ic_stride or est
+    nearly = (early // early_every) * ((N + est - 1) // est) + (N + ist - 1) // ist if early > 0 else 0
+    ctx.mlp_reserve(nsnap * per + nearly)
     t0 = time.time()
+    if early > 0:
+        for s in range(early + 1):
+            if s:
+                ctx.step()
+            if s == 0:
+                ctx.mlp_collect(ist, offset=0)
+            elif s % early_every == 0:
+                ctx.mlp_collect(est, offset=(s * 104729) % est)
     for s in range(warmup):
         ctx.step()
         if (s + 1) % 500 == 0:
             log("  Mc %.2f Re %.0f warmup step %d, dt %.3e" % (mach, reynolds, s + 1, ctx.dt))
-    sizes = []
+    n_early = ctx.mlp_sample_count()
+    sizes = [n_early] if early > 0 else []
     for q in range(nsnap):
         for _ in range(every):
             ctx.step()
@@ -61,27 +80,51 @@ def generate(mesh, mach, reynolds, warmup, steps, every, stride, seed):
         sizes.append(ctx.mlp_sample_count() - before)
     X, t = ctx.mlp_samples()
     st = shear_case.statistics(ctx, case, u_in)
-    log("  Mc %.2f Re %.0f: %d steps, %d samples in %.1f s; delta_theta %.4g, tke %.4g, Cd>0 in %.1f %%"
-        % (mach, reynolds, warmup + nsnap * every, len(t), time.time() - t0, st["delta_theta"], st["tke"],
-           100 * np.mean(t > 0)))
+    log("  Mc %.2f Re %.0f pert %.2f: %d steps, %d samples (%d early) in %.1f s; delta_theta %.4g, tke %.4g, "
+        "Cd>0 in %.1f %%" % (mach, reynolds, pert, warmup + nsnap * every + early, len(t), n_early,
+                            time.time() - t0, st["delta_theta"], st["tke"], 100 * np.mean(t > 0)))
     ctx.mlp_clear_samples()
     return ctx, X, t, np.array(sizes)
 
 
-def deploy_check(params, norm, mesh, mach, reynolds, steps, seed):
+def deploy_check(params, norm, mesh, mach, reynolds, pert, steps, seed, lcg_seed):
     """The closure as deployed (judge every 50 steps with the dynamic
     fallback and the judge's global rescale) from the initial state at
-    another flow condition; returns the judge errors and the status."""
+    another flow condition; returns the judges' level and profile errors,
+    the status and the shear-layer statistics of the final state."""
     L, H, D = mesh
     ctx, case, U, u_in, _ = shear_case.make(L, H, D, sgs="learned", precision="mixed", mach=mach,
-                                            reynolds=reynolds, seed=seed)
+                                            reynolds=reynolds, pert=pert, seed=seed, lcg_seed=lcg_seed)
     ctx.mlp_set_params(params, norm)
     ctx.learned_configure(50, 0.10)
+    ctx.learned_set_tolerances(1e30, 1e30)   # scored here, not enforced
     ctx.run(steps)
-    err, _ = ctx.learned_history()
+    err, perr, _, _ = ctx.learned_history(errors=True)
     st = ctx.learned_status()
+    stats = shear_case.statistics(ctx, case, u_in)
     ctx.close()
-    return [float(e) for e in err], st
+    return [float(e) for e in err], [float(e) for e in perr], st, stats
+
+
+def reference_statistics(mesh, mach, reynolds, pert, steps, seed, lcg_seed):
+    """Shear-layer statistics of the FP64 dynamic-model run of a deployment
+    check's case after the same number of steps."""
+    L, H, D = mesh
+    ctx, case, U, u_in, _ = shear_case.make(L, H, D, sgs="dynamic", precision="fp64", mach=mach,
+                                            reynolds=reynolds, pert=pert, seed=seed, lcg_seed=lcg_seed)
+    ctx.run(steps)
+    stats = shear_case.statistics(ctx, case, u_in)
+    ctx.close()
+    return {k: float(stats[k]) for k in ("delta_theta", "delta_theta_out", "delta_w", "tke")}
+
+
+def parse_conditions(spec, mesh):
+    """'Mc:Re:pert[:LxHxD],...' -> [(Mc, Re, pert, mesh)], the mesh defaulting to `mesh`."""
+    out = []
+    for item in spec.split(","):
+        v = item.split(":")
+        out.append((float(v[0]), float(v[1]), float(v[2]), shear_case.parse_mesh(v[3]) if len(v) > 3 else mesh))
+    return out
 
 
 def main(argv=None):
@@ -89,8 +132,16 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     ap.add_argument("--report")
     ap.add_argument("--mesh", default="192x128x96")
-    ap.add_argument("--mach", default="0.3,0.5")
-    ap.add_argument("--reynolds", default="500,2000", help="Re on delta_w0 of the training runs")
+    ap.add_argument("--runs", default="0.3:500:0.02,0.5:500:0.02,0.3:2000:0.02,0.5:2000:0.02,0.5:2000:0.03",
+                    help="Mc:Re:perturbation[:mesh] of the training runs (Re on delta_w0; mesh default --mesh)")
+    ap.add_argument("--early", type=int, default=40,
+                    help="also sample the initial state and each of the first N steps of every run")
+    ap.add_argument("--early-every", type=int, default=1, help="step spacing of the early samples")
+    ap.add_argument("--early-stride", type=int, default=0, help="cell stride of the early samples (0: auto)")
+    ap.add_argument("--ic-stride", type=int, default=0,
+                    help="cell stride on the initial state (0: the early stride)")
+    ap.add_argument("--profile-tol", type=float, default=0.25,
+                    help="profile-error bound of the judge, used in the deployment score")
     ap.add_argument("--samples", type=int, default=10_000_000, help="total samples (all runs)")
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--steps", type=int, default=3000, help="collection steps per run")
@@ -103,36 +154,64 @@ def main(argv=None):
                     help="candidate loss weights (|S| Delta / dU)^p")
     ap.add_argument("--seeds", type=int, default=3, help="initialisations per weight power")
     ap.add_argument("--no-calibrate", dest="calibrate", action="store_false")
-    ap.add_argument("--eval", default="0.4:500,0.6:2000", help="Mc:Re of the deployment checks")
+    ap.add_argument("--eval", default="0.4:500:0.02,0.6:2000:0.03",
+                    help="Mc:Re:perturbation[:mesh] of the deployment checks")
     ap.add_argument("--eval-steps", type=int, default=1000, help="steps of each deployment check")
+    ap.add_argument("--eval-seed", type=int, default=4242, help="perturbation seed of the deployment checks")
+    ap.add_argument("--eval-lcg", type=float, default=0.5772156649,
+                    help="disturbance-block generator seed of the deployment checks")
+    ap.add_argument("--tke-tol", type=float, default=0.02,
+                    help="bound on the final resolved TKE and delta_theta deviation of a deployment check")
     a = ap.parse_args(argv)
 
     mesh = shear_case.parse_mesh(a.mesh)
-    conds = [(float(m), float(re)) for m in a.mach.split(",") for re in a.reynolds.split(",")]
-    evals = [tuple(float(v) for v in e.split(":")) for e in a.eval.split(",")] if a.eval else []
-    N = mesh[0] * mesh[1] * mesh[2]
+    conds = parse_conditions(a.runs, mesh)
+    evals = parse_conditions(a.eval, mesh) if a.eval else []
     nsnap = a.steps // a.every
-    stride = max(1, int(np.ceil(N * nsnap * len(conds) / a.samples)))
-    log("mesh %s, (Mc, Re) %s: %d snapshots per run, stride %d" % (a.mesh, conds, nsnap, stride))
+    n_early = a.early // a.early_every + 1 if a.early else 0
+    N0 = mesh[0] * mesh[1] * mesh[2]
+    per_run = a.samples / len(conds)
+    log("(Mc, Re, pert, mesh) %s: %d snapshots per run, %d early states" % (conds, nsnap, n_early))
 
-    runs, Xtr, ttr, Xva, tva = [], [], [], [], []
-    for r, (mach, reynolds) in enumerate(conds):
-        ctx, X, t, sizes = generate(mesh, mach, reynolds, a.warmup, a.steps, a.every, stride, seed=1000 + r)
+    runs, Xtr, ttr, Xva, tva, strides = [], [], [], [], [], []
+    for r, (mach, reynolds, pert, m) in enumerate(conds):
+        N = m[0] * m[1] * m[2]
+        stride = max(1, int(np.ceil(N * nsnap / per_run)))
+        # the early states: --early-stride on the --mesh (scaled with the cell
+        # count on other meshes), else about a fifth of the run's samples
+        if not a.early:
+            est = 0
+        elif a.early_stride:
+            est = max(1, int(round(a.early_stride * N / N0)))
+        else:
+            est = max(1, int(np.ceil(5 * N * n_early / (N * nsnap / stride))))
+        strides.append((stride, est))
+        ctx, X, t, sizes = generate(m, mach, reynolds, pert, a.warmup, a.steps, a.every, stride, seed=1000 + r,
+                                    early=a.early, early_stride=est, early_every=a.early_every,
+                                    ic_stride=max(1, int(round(a.ic_stride * N / N0))) if a.ic_stride else None)
         nva = int(round(a.holdout * len(sizes)))
         cut = int(sizes[:len(sizes) - nva].sum())
         Xtr.append(X[:cut]); ttr.append(t[:cut])
         Xva.append(X[cut:]); tva.append(t[cut:])
-        runs.append((mach, reynolds, ctx))
+        runs.append((mach, reynolds, pert, ctx))
     Xtr = np.concatenate(Xtr); ttr = np.concatenate(ttr)
     log("training samples %d, validation samples %d" % (len(ttr), sum(len(v) for v in tva)))
+    # the FP64 dynamic-model statistics each deployment check is compared with
+    refs = []
+    for mach_e, re_e, pert_e, m in evals:
+        refs.append(reference_statistics(m, mach_e, re_e, pert_e, a.eval_steps, a.eval_seed, a.eval_lcg))
+        log("  reference Mc %.2f Re %.0f %s after %d steps: delta_theta %.4g, tke %.4g"
+            % (mach_e, re_e, "x".join(map(str, m)), a.eval_steps, refs[-1]["delta_theta"], refs[-1]["tke"]))
 
-    trainer = runs[0][2]
+    trainer = runs[0][3]
     trainer.mlp_load_samples(Xtr, ttr)
     norm0 = trainer.mlp_fit_norm()
     wtr = Xtr[:, 6].astype(np.float64) ** 3   # share of the SGS dissipation rho Cd Delta^2 |S|^3
-    report = {"mesh": a.mesh, "conditions": conds, "stride": stride, "warmup": a.warmup, "steps": a.steps,
-              "every": a.every, "train_samples": int(len(ttr)), "batch": a.batch, "schedule": a.schedule,
-              "target_variance": float(np.var(ttr)), "candidates": []}
+    report = {"conditions": [(c[0], c[1], c[2], "x".join(map(str, c[3]))) for c in conds],
+              "strides": strides, "warmup": a.warmup, "steps": a.steps, "every": a.every, "early": a.early,
+              "early_every": a.early_every, "profile_tol": a.profile_tol, "tke_tol": a.tke_tol,
+              "train_samples": int(len(ttr)), "batch": a.batch, "schedule": a.schedule,
+              "target_variance": float(np.var(ttr)), "eval_references": refs, "candidates": []}
     best = None
     cands = [(float(v), a.seed + k) for v in a.weight_power.split(",") for k in range(a.seeds)]
     for wp, seed in cands:
@@ -155,49 +234,62 @@ def main(argv=None):
         params[4736:4801] *= alpha
         cand = {"weight_power": wp, "seed": seed, "epoch_loss": losses, "train_seconds": train_s, "alpha": alpha,
                 "runs": []}
-        for (mach, reynolds, ctx), Xv, tv in zip(runs, Xva, tva):
+        for (mach, reynolds, pert, ctx), Xv, tv in zip(runs, Xva, tva):
             ctx.mlp_set_params(params, norm)
             yv = np.clip(ctx.mlp_forward(Xv), 0.0, 1.0)
             wv = Xv[:, 6].astype(np.float64) ** 3
             mse = float(np.mean((yv - tv) ** 2))
             bias = float((wv * (yv - tv)).sum() / max((wv * tv).sum(), 1e-30))
             ctx.ghost_fill()
-            err, dd, dl = ctx.learned_judge()
-            cand["runs"].append({"mach": mach, "reynolds": reynolds, "val_samples": int(len(tv)), "val_mse": mse,
-                                 "val_r2": 1 - mse / max(float(np.var(tv)), 1e-30),
-                                 "val_dissipation_bias": bias, "judge_error": err,
+            err, perr, _, dd, dl = ctx.learned_judge(errors=True)
+            cand["runs"].append({"mach": mach, "reynolds": reynolds, "pert": pert, "val_samples": int(len(tv)),
+                                 "val_mse": mse, "val_r2": 1 - mse / max(float(np.var(tv)), 1e-30),
+                                 "val_dissipation_bias": bias, "judge_error": err, "judge_profile_error": perr,
                                  "dissipation_dynamic": dd, "dissipation_learned": dl})
         cand["deploy"] = []
         fails, errs = 0, []
-        for mach_e, re_e in evals:
-            dep, dst = deploy_check(params, norm, mesh, mach_e, re_e, a.eval_steps, seed=4242)
-            cand["deploy"].append({"mach": mach_e, "reynolds": re_e, "every": 50, "judge_error": dep, "status": dst})
-            # the synthetic initial state is left to the fallback: its judge is not scored
-            fails += sum(e > 0.10 for e in dep[1:])
-            errs += dep[1:]
+        for (mach_e, re_e, pert_e, m), ref in zip(evals, refs):
+            dep, pdep, dst, stats = deploy_check(params, norm, m, mach_e, re_e, pert_e, a.eval_steps,
+                                                 a.eval_seed, a.eval_lcg)
+            dtke = stats["tke"] / ref["tke"] - 1
+            dth = stats["delta_theta"] / ref["delta_theta"] - 1
+            cand["deploy"].append({"mach": mach_e, "reynolds": re_e, "pert": pert_e, "mesh": "x".join(map(str, m)),
+                                   "every": 50, "judge_error": dep, "judge_profile_error": pdep, "status": dst,
+                                   "tke_rel": dtke, "delta_theta_rel": dth})
+            # every judge scores, the first (synthetic initial state) included: a judge
+            # fails on the level error (0.10) or the profile error (profile_tol);
+            # the final resolved TKE and delta_theta fail beyond tke_tol of the
+            # dynamic model's
+            fails += sum(e > 0.10 or p > a.profile_tol for e, p in zip(dep, pdep))
+            fails += int(abs(dtke) > a.tke_tol) + int(abs(dth) > a.tke_tol)
+            errs += [e + p for e, p in zip(dep, pdep)]
         score = (fails, float(np.mean(errs)) if errs else 0.0)
         cand["score"] = list(score)
         log("  weight power %.1f seed %d: %.1f s, final loss %.3e, alpha %.3f; %s" % (
             wp, seed, train_s, losses[-1], alpha, "; ".join(
-                "Mc %.2f Re %.0f R^2 %.3f judge %.3f" % (r["mach"], r["reynolds"], r["val_r2"], r["judge_error"])
+                "Mc %.2f Re %.0f R^2 %.3f judge %.3f/%.3f" % (r["mach"], r["reynolds"], r["val_r2"], r["judge_error"],
+                                                          r["judge_profile_error"])
                 for r in cand["runs"])))
         for d in cand["deploy"]:
-            log("    deployment Mc %.2f Re %.0f: %d of %d later judges failed; errors %s" % (
-                d["mach"], d["reynolds"], sum(e > 0.10 for e in d["judge_error"][1:]), len(d["judge_error"]) - 1,
-                " ".join("%.3f" % e for e in d["judge_error"])))
+            log("    deployment Mc %.2f Re %.0f %s: tke %+.2f %%, delta_theta %+.2f %%; %d of %d judges failed; "
+                "errors %s; profile errors %s" % (
+                    d["mach"], d["reynolds"], d["mesh"], 100 * d["tke_rel"], 100 * d["delta_theta_rel"],
+                    sum(e > 0.10 or p > a.profile_tol for e, p in zip(d["judge_error"], d["judge_profile_error"])),
+                    len(d["judge_error"]), " ".join("%.3f" % e for e in d["judge_error"]),
+                    " ".join("%.3f" % e for e in d["judge_profile_error"])))
         report["candidates"].append(cand)
         if best is None or score < best[0]:
             best = (score, wp, seed, params, norm)
     trainer.mlp_clear_samples()
     score, wp, seed, params, norm = best
     report["selected"] = {"weight_power": wp, "seed": seed, "score": list(score)}
-    log("selected weight power %.1f seed %d (%d failed later judges, mean error %.3f)" % (wp, seed, score[0], score[1]))
+    log("selected weight power %.1f seed %d (%d failed judges, mean error %.3f)" % (wp, seed, score[0], score[1]))
     fsl.mlp_file_write(a.out, params, norm)
     log("wrote", a.out)
     if a.report:
         with open(a.report, "w") as f:
             json.dump(report, f, indent=1)
-    for _, _, ctx in runs:
+    for *_, ctx in runs:
         ctx.close()
     return 0
 

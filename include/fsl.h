@@ -388,15 +388,21 @@ int fsl_mlp_train(fsl_ctx *ctx, int epochs, long long batch, float lr, unsigned 
    loss receives the sum of weighted squared errors */
 int fsl_mlp_gradient(fsl_ctx *ctx, long long batch, float *grad, float *loss);
 /* Judge: every interval steps (steps 1, 1 + interval, ...) both models run
-   and the relative difference of the SGS dissipation sum(mu_SGS |S|^2) is
-   the error; above tol the dynamic model is used, and kept every step until
-   a later judge passes.  With adapt on, each judge also sets the network's
+   and two errors of the network's mu_SGS against the dynamic model's are
+   formed: the relative difference of the SGS dissipation D = sum(mu_SGS
+   |S|^2) over the interior, and the profile error || s P_l - P_d ||_2 /
+   || P_d ||_2 of the y profiles P(j) of the dissipation (x-z sums of each
+   row) with s = D_d / D_l, which measures its distribution across the layer
+   once the level is matched.  When the first exceeds tol or the second
+   profile_tol (fsl_learned_set_profile_tol, default 0.25) the dynamic model
+   is used, and kept every step until a later judge passes.  With adapt on, each judge also sets the network's
    output scale (Cd = clip(0.15 scale y, 0, 0.15)) to scale D_dynamic /
    D_learned, clamped to [0.1, 10], for the following steps: the network
    gives the spatial distribution of Cd, the dynamic procedure its global
    level.  Defaults: interval 50, tol 0.10, scale 1, adapt on; interval 0
    never judges. */
 int fsl_learned_configure(fsl_ctx *ctx, int interval, double tol);
+int fsl_learned_set_tolerances(fsl_ctx *ctx, double profile_tol, double local_tol);
 int fsl_learned_set_scale(fsl_ctx *ctx, double scale, int adapt);
 typedef struct fsl_learned_status {
     int interval;
@@ -406,12 +412,20 @@ typedef struct fsl_learned_status {
     long long judged, fallbacks, learned_steps, dynamic_steps;
     double scale;            /* output scale in use */
     int adapt;
+    double profile_tol, local_tol;
+    double last_profile_error;   /* -1 before the first judge */
+    double last_local_error;
 } fsl_learned_status;
 int fsl_learned_get_status(const fsl_ctx *ctx, fsl_learned_status *s);
 /* judge history: error and the scale in use at each judge; returns the count */
 int fsl_learned_history(const fsl_ctx *ctx, int nmax, double *err, double *scale);
+/* the same with the profile and local errors (any pointer may be NULL) */
+int fsl_learned_history_errors(const fsl_ctx *ctx, int nmax, double *err, double *profile_err, double *local_err,
+                               double *scale);
 /* one judge evaluation on U now (no state change beyond mu_SGS = dynamic) */
 int fsl_learned_judge(fsl_ctx *ctx, double *err, double *d_dynamic, double *d_learned);
+int fsl_learned_judge_errors(fsl_ctx *ctx, double *err, double *profile_err, double *local_err,
+                             double *d_dynamic, double *d_learned);
 /* one v_mfma_f32_16x16x16_bf16 product D = A B of row-major 16x16 FP32
    matrices rounded to BF16 (operand layout check) */
 int fsl_mfma_probe(fsl_ctx *ctx, const float *A, const float *B, float *D);
