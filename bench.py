@@ -2,8 +2,8 @@
 
 Default workload: the FP32 + learned-closure path (mixed precision: FP32
 stencil arithmetic, FP64 state / pressure recovery / RK / CFL; the per-cell
-network for Cs^2 on the matrix cores (FP16 operands, FP32 accumulation;
-trained in BF16), judged against the dynamic model every 50
+network for Cs^2 -- the shipped model, ReLU hidden layers -- on the matrix
+cores (FP16 operands, FP32 accumulation; trained in BF16), judged against the dynamic model every 50
 steps with a dynamic fallback above 10 % error) on the synthetic free shear
 layer at 384x256x256 cells, Mc 0.4 -- tanh mean profile U(y) = Um + dU/2
 tanh(2y/delta_w0), U2/U1 = 0.5, Re 500 on delta_w0 = 4 dy, seeded 2 % dU
@@ -37,7 +37,7 @@ reported as `cpu_port`) -- a reported baseline, not the target.
 
     python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--mesh 384x256x256]
                     [--sgs learned|dynamic|constant] [--precision mixed|fp64]
-                    [--mach 0.4] [--path fused|unfused] [--no-compare] [--no-cpu]
+                    [--mach 0.4] [--path fused|unfused] [--model file] [--no-compare] [--no-cpu]
 """
 import argparse
 import json
@@ -106,9 +106,11 @@ def host_cpu():
     return model, os.cpu_count()
 
 
-def workload_name(precision, sgs, path, L, H, D, mach):
-    return "%s %s-SGS %s path, %dx%dx%d cells, Mc %.1f, 3-stage TVD RK" % (
-        "FP32+FP64" if precision == "mixed" else "FP64", sgs, path, L, H, D, mach)
+def workload_name(precision, sgs, path, L, H, D, mach, activation=None):
+    net = " (%s network)" % {fsl.FSL_MLP_TANH: "tanh", fsl.FSL_MLP_RELU: "ReLU"}[activation] \
+        if sgs == "learned" and activation is not None else ""
+    return "%s %s-SGS%s %s path, %dx%dx%d cells, Mc %.1f, 3-stage TVD RK" % (
+        "FP32+FP64" if precision == "mixed" else "FP64", sgs, net, path, L, H, D, mach)
 
 
 def spin_up(ctx, cap):
@@ -133,13 +135,12 @@ WINDOW_KEYS = ("judged", "fallbacks", "learned_steps", "dynamic_steps")
 
 
 def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=True, spinup_cap=400,
-             presteps=0):
+             presteps=0, model=MODEL):
     ctx, case, U0, u_in, _ = shear_case.make(L, H, D, sgs=sgs, precision=precision, fused=(path == "fused"),
                                              cfl=cfl, mach=mach)
     spun = 0
     if sgs == "learned":
-        p, n = fsl.mlp_file_read(MODEL)
-        ctx.mlp_set_params(p, n)
+        ctx.mlp_load(model)
         ctx.learned_configure(50, 0.10)
         spun = spin_up(ctx, spinup_cap)
     for _ in range(presteps + warmup):
@@ -171,8 +172,8 @@ def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=Tr
 # Sources whose text determines each profiled kernel's machine code: a
 # counter figure recorded for other sources is stale and is not reported.
 KERNEL_SOURCES = {
-    ("fused_stage", "mixed"): ["fsl_fused_mixed.hip", "fsl_device_mixed.hpp", "fsl_device.hpp"],
-    ("fused_stage", "fp64"): ["fsl_fused.hip", "fsl_device.hpp"],
+    ("fused_stage", "mixed"): ["fsl_fused_mixed.hip", "fsl_device_mixed.hpp", "fsl_device.hpp", "fsl_buffer.hpp"],
+    ("fused_stage", "fp64"): ["fsl_fused.hip", "fsl_device.hpp", "fsl_buffer.hpp"],
     ("mlp_mu", "mixed"): ["fsl_mlp.hip"],
     ("mlp_mu", "fp64"): ["fsl_mlp.hip"],
 }
@@ -299,6 +300,7 @@ def main(argv=None):
     ap.add_argument("--path", default="fused", choices=["fused", "unfused"])
     ap.add_argument("--mach", type=float, default=0.4)
     ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--model", default=MODEL, help="learned-closure model file (FSLMLP01 / FSLMLP02)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 dynamic comparison leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -320,10 +322,11 @@ def main(argv=None):
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         return 2
     L, H, D = shear_case.parse_mesh(a.mesh)
-    wl = workload_name(a.precision, a.sgs, a.path, L, H, D, a.mach)
+    act = fsl.mlp_file_read_act(a.model)[2] if a.sgs == "learned" else None
+    wl = workload_name(a.precision, a.sgs, a.path, L, H, D, a.mach, act)
     print("[bench] %s: %d warmup + %d timed steps" % (wl, a.warmup, a.steps), file=sys.stderr, flush=True)
     leg = time_leg(L, H, D, a.sgs, a.precision, a.path, a.cfl, a.mach, a.steps, a.warmup, not a.no_profile,
-                   spinup_cap=a.spinup_cap)
+                   spinup_cap=a.spinup_cap, model=a.model)
     ctx = leg["ctx"]
     failures = []
     if not np.all(np.isfinite(leg["sums"])):
@@ -381,7 +384,10 @@ def main(argv=None):
         "data": "synthetic shear layer (tanh profile, Mc %.1f, U2/U1 0.5, Re 500, 2%% seeded perturbations, "
                 "acoustic CFL %.2f)" % (a.mach, a.cfl),
         "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
-                   "path": a.path, "mach": a.mach},
+                   "path": a.path, "mach": a.mach,
+                   "closure": ({"model": os.path.relpath(a.model, ROOT),
+                                "activation": {fsl.FSL_MLP_TANH: "tanh", fsl.FSL_MLP_RELU: "relu"}[act]}
+                               if a.sgs == "learned" else None)},
         "roofline": roofline,
         "mlp_roofline": mlp_roofline,
         "fp64_dynamic": compare,

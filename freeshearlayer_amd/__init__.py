@@ -23,6 +23,8 @@ FSL_ERR_ARG, FSL_ERR_HIP, FSL_ERR_NOMEM, FSL_ERR_NONFINITE, FSL_ERR_IO, FSL_ERR_
 FSL_SGS_CONSTANT, FSL_SGS_DYNAMIC, FSL_SGS_LEARNED = 0, 1, 2
 FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
 FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1
+FSL_MLP_TANH, FSL_MLP_RELU = 0, 1
+MLP_ACTIVATIONS = {"tanh": FSL_MLP_TANH, "relu": FSL_MLP_RELU}
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -160,6 +162,10 @@ def load():
         "fsl_mlp_init_params": (None, [C.c_ulonglong, _fp, _fp]),
         "fsl_mlp_file_write": (i_, [cc, _fp, _fp]),
         "fsl_mlp_file_read": (i_, [cc, _fp, _fp]),
+        "fsl_mlp_file_write_act": (i_, [cc, _fp, _fp, i_]),
+        "fsl_mlp_file_read_act": (i_, [cc, _fp, _fp, C.POINTER(i_)]),
+        "fsl_mlp_set_activation": (i_, [vp, i_]),
+        "fsl_mlp_get_activation": (i_, [vp]),
         "fsl_mlp_set_params": (i_, [vp, _fp, _fp]),
         "fsl_mlp_get_params": (i_, [vp, _fp, _fp]),
         "fsl_mlp_forward": (i_, [vp, _fp, ll, _fp]),
@@ -423,6 +429,20 @@ class Context:
             raise ValueError("expected %d parameters and 16 normalisation values" % MLP_NPARAM)
         self._chk(self.lib.fsl_mlp_set_params(self.h, _fptr(p), _fptr(n)), "mlp_set_params")
 
+    def mlp_set_activation(self, activation):
+        """Hidden activation of the network: FSL_MLP_TANH / FSL_MLP_RELU or 'tanh' / 'relu'."""
+        act = MLP_ACTIVATIONS.get(activation, activation)
+        self._chk(self.lib.fsl_mlp_set_activation(self.h, int(act)), "mlp_set_activation")
+
+    def mlp_activation(self):
+        return self.lib.fsl_mlp_get_activation(self.h)
+
+    def mlp_load(self, path):
+        """Parameters, normalisation and activation from a model file."""
+        p, n, act = mlp_file_read_act(path)
+        self.mlp_set_params(p, n)
+        self.mlp_set_activation(act)
+
     def mlp_get_params(self):
         p = np.zeros(MLP_NPARAM, np.float32)
         n = np.zeros(16, np.float32)
@@ -538,6 +558,27 @@ class Context:
             return e.value, pe.value, le.value, dd.value, dl.value
         return e.value, dd.value, dl.value
 
+    def output_fields(self, k0=1, nk=None):
+        """Output()'s nine computed columns (rho, u, v, w, p, T, |omega|, Q,
+        mu_T/mu_L) of planes k0 .. k0+nk-1: array (nk, HIG, LEN, 9)."""
+        case = self.case
+        if nk is None:
+            nk = case.DEP - k0 + 1
+        out = np.zeros((nk, case.HIG, case.LEN, 9))
+        self._chk(self.lib.fsl_output_fields(self.h, k0, nk, _ptr(out)), "output_fields")
+        return out
+
+    def write_plt(self, path=None):
+        self._chk(self.lib.fsl_write_plt(self.h, path.encode() if path else None), "write_plt")
+
+    def probe_values(self):
+        out = np.zeros((6, self.case.HIG))
+        self._chk(self.lib.fsl_probe_values(self.h, _ptr(out)), "probe_values")
+        return out
+
+    def write_probes(self, path=None, real_bytes=8):
+        self._chk(self.lib.fsl_write_probes(self.h, path.encode() if path else None, real_bytes), "write_probes")
+
     def mfma_probe(self, A, B):
         A = _f32(A, (16, 16))
         B = _f32(B, (16, 16))
@@ -615,6 +656,25 @@ def mlp_file_read(path):
     if rc != 0:
         raise FslError("fsl_mlp_file_read(%s) returned %d" % (path, rc))
     return p, n
+
+
+def mlp_file_write_act(path, params, norm, activation):
+    p, n = _f32(params), _f32(norm)
+    act = MLP_ACTIVATIONS.get(activation, activation)
+    rc = load().fsl_mlp_file_write_act(path.encode(), _fptr(p), _fptr(n), int(act))
+    if rc != 0:
+        raise FslError("fsl_mlp_file_write_act(%s) returned %d" % (path, rc))
+
+
+def mlp_file_read_act(path):
+    """(params, norm, activation) of a model file of either format."""
+    p = np.zeros(MLP_NPARAM, np.float32)
+    n = np.zeros(16, np.float32)
+    act = C.c_int(0)
+    rc = load().fsl_mlp_file_read_act(path.encode(), _fptr(p), _fptr(n), C.byref(act))
+    if rc != 0:
+        raise FslError("fsl_mlp_file_read_act(%s) returned %d" % (path, rc))
+    return p, n, act.value
 
 
 def ic_reference(case):

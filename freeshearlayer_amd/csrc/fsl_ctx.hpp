@@ -30,6 +30,7 @@ struct LearnedState {
     int adam_t = 0;
     float wpow = 0.f, winv = 1.f;   // loss weight (feature 6)^wpow, normalised to mean 1
     int ready = 0;             // parameters set
+    int act = 0;               // hidden activation, FSL_MLP_TANH or FSL_MLP_RELU
     float* X = nullptr;        // sample store: features (8 per sample), targets
     float* t = nullptr;
     long long n = 0, cap = 0;

@@ -37,10 +37,24 @@ struct MixedArgs {
 
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 
+// v of the lane after / before this one in its row of 16 lanes (DPP
+// row_shl:1 / row_shr:1 on both halves; the row's last / first lane gets 0).
+// Every lane of the wave must execute these.
+template <int CTRL>
+__device__ __forceinline__ double row_dpp(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double row_from_next(double v) { return row_dpp<0x101>(v); }
+__device__ __forceinline__ double row_from_prev(double v) { return row_dpp<0x111>(v); }
+
 template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 {
-    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK, WK = BK + 2, NW = (BJ + 2) * WK;
+    constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK, NW = (BJ + 2) * BK;
     __shared__ float s_vr[3][3][NT];    // 1/rho velocities, planes i-1, i, i+1
     __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
     __shared__ float s_ym[4][NT];       // - face states along y: density, momenta
@@ -50,8 +64,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
     __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
-    __shared__ double s_w[2][5][NW];    // state of planes i, i+1 (slot p & 1) with a ring of outer
-                                        // neighbours: y/z neighbours
+    __shared__ double s_w[2][5][NW];    // state of planes i, i+1 (slot p & 1) with the rows of outer
+                                        // neighbours above and below the tile: y neighbours
 
     const Geo& g = a.g;
     const int L = g.L, H = g.H, D = g.D;
@@ -92,15 +106,19 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     // columns (colmu).
     const int kim = ((k - 1) % D + D) % D + 1;
     const unsigned colb = (unsigned)jc * sjb + (unsigned)kim * 8u, colmu = (unsigned)jc * sjb + (unsigned)kc * 8u;
-    // y/z neighbours of a lane's cell come from s_w: the cells of the other
-    // lanes and a ring of outer neighbours that the edge rows and columns of
-    // the tile load (one load set per direction: a lane lacks at most one y
-    // and one z side).  In y the rows are clamped (j - 1 >= 0, j + 1 <= H + 1),
-    // in z the columns periodic; every neighbour a used face state needs is
-    // the one the reference's ghost layer holds there.
-    const int cw = (ty + 1) * WK + tx + 1;
+    // Neighbours of a lane's cell in its plane.  z: the cells of the lanes
+    // beside it in its row of BK = 16 lanes (one DPP row), moved across by
+    // row shifts of the registers; the row's end lanes load the outer
+    // neighbour.  y: the cells of the rows above and below through s_w, a
+    // copy of the plane in 16-double rows (two rows of one half-wave fill all
+    // banks once) with a row of outer neighbours above and below the tile
+    // that the tile's edge rows load.  In y the rows are clamped (j - 1 >= 0,
+    // j + 1 <= H + 1), in z the columns periodic; every neighbour a used face
+    // state needs is the one the reference's ghost layer holds there.
+    static_assert(BK == 16, "z neighbours move along one 16-lane DPP row");
+    const int cw = (ty + 1) * BK + tx;
     const bool yedge = ty == 0 || ty == BJ - 1, zedge = tx == 0 || tx == BK - 1;
-    const int cwy = ty == 0 ? cw - WK : cw + WK, cwz = tx == 0 ? cw - 1 : cw + 1;
+    const int cwy = ty == 0 ? cw - BK : cw + BK;
     const int kie = (((tx == 0 ? k - 1 : k + 1) - 1) % D + D) % D + 1;
     const unsigned oye = (unsigned)(ty == 0 ? jm : jp) * sjb + (unsigned)kim * 8u;
     const unsigned oze = (unsigned)jc * sjb + (unsigned)kie * 8u;
@@ -111,6 +129,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
     double w0[5], w1[5], w2[5];
     double pye[5], pze[5];                   // edge lanes: outer y/z neighbours of the next plane
+#pragma unroll
+    for (int f = 0; f < 5; f++) pze[f] = 0.;
     float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
     float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
     mx::FaceF xp_prev;
@@ -149,17 +169,19 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // step state of the cell this iteration updates (i - 1)
         double bq[5];
         if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, colb, bq);
-        // y and z neighbours of this plane's cell, written to s_w in the
-        // previous iteration
+        // y neighbours of this plane's cell (written to s_w in the previous
+        // iteration); z neighbours from the lanes beside it, and at a row's
+        // ends the outer neighbour loaded in the previous iteration
         double nyp[5], nym[5], nzp[5], nzm[5];
         {
             const int sl = i & 1;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
-                nyp[f] = s_w[sl][f][cw + WK];
-                nym[f] = s_w[sl][f][cw - WK];
-                nzp[f] = s_w[sl][f][cw + 1];
-                nzm[f] = s_w[sl][f][cw - 1];
+                nyp[f] = s_w[sl][f][cw + BK];
+                nym[f] = s_w[sl][f][cw - BK];
+                const double zn = row_from_next(w1[f]), zb = row_from_prev(w1[f]);   // all lanes
+                nzp[f] = tx == BK - 1 ? pze[f] : zn;
+                nzm[f] = tx == 0 ? pze[f] : zb;
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
             if (yedge) load5<PK>(a.in, pn, pb, fb, oye, pye);
@@ -350,15 +372,11 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
         }
-        // outer neighbours of plane i + 1 (the slot's last reads were the
+        // outer y neighbours of plane i + 1 (the slot's last reads were the
         // previous plane's, before its barriers)
         if (yedge) {
 #pragma unroll
             for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cwy] = pye[f];
-        }
-        if (zedge) {
-#pragma unroll
-            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cwz] = pze[f];
         }
         lds_barrier();
         if (inseg && interior) {

@@ -84,19 +84,20 @@ int fused_segments(int L, int cols, int per_cu, int* seg_len);
 // --- learned closure (fsl_mlp.hip) ---
 int mlp_nparam();
 int mlp_npack();   // packed BF16 operand entries (8 bytes each)
-void launch_mlp_pack(hipStream_t s, const float* master, void* pack);
+// act: FSL_MLP_TANH or FSL_MLP_RELU, the network's hidden activation
+void launch_mlp_pack(hipStream_t s, const float* master, void* pack, int act);
 void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* pack, const float* master,
-                        const float* norm, float* y);
+                        const float* norm, float* y, int act);
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                    double dy, const void* pack, const float* master, const float* norm, double* mu,
-                   float* cd_out, float scale);
+                   float* cd_out, float scale, int act);
 void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                          double dy, const float* cd, long long stride, long long offset, float* X, float* t,
                          long long nout);
 void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
                            long long step, unsigned long long pa, unsigned long long pc, const void* pack,
                            const float* master, const float* norm, float* grad, float* loss, float wpow,
-                           float winv);
+                           float winv, int act);
 void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t);
 void launch_mfma_probe(hipStream_t s, const float* A, const float* B, float* D);
 // per-row (y) sums over x band s of the SGS dissipation densities e_a, e_b

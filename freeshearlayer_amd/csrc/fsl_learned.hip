@@ -121,7 +121,7 @@ int learned_mu(fsl_ctx* c, double* mu)
     }
     prof_mark(c, K_MLP_MU, 0);
     launch_mlp_mu(c->stream, cst5(c->U, c->g.G), c->g, c->ph, c->dU, L.dw, c->cs.deltaY, L.pack, L.master,
-                  L.norm, mu, nullptr, L.scale);
+                  L.norm, mu, nullptr, L.scale, L.act);
     prof_mark(c, K_MLP_MU, 1);
     return check_launch(c, "learned mu_SGS");
 }
@@ -294,7 +294,7 @@ int fsl_mlp_set_params(fsl_ctx* c, const float* params, const float* norm)
     LearnedState& L = c->lrn;
     HIPCHK(c, hipMemcpyAsync(L.master, params, mlp_nparam() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(L.norm, norm, 16 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    launch_mlp_pack(c->stream, L.master, L.pack);
+    launch_mlp_pack(c->stream, L.master, L.pack, L.act);
     rc = check_launch(c, "mlp pack");
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -310,6 +310,24 @@ int fsl_mlp_set_params(fsl_ctx* c, const float* params, const float* norm)
     }
     return FSL_OK;
 }
+
+int fsl_mlp_set_activation(fsl_ctx* c, int activation)
+{
+    if (activation != FSL_MLP_TANH && activation != FSL_MLP_RELU)
+        return set_err(c, FSL_ERR_ARG, "activation %d", activation);
+    LearnedState& L = c->lrn;
+    L.act = activation;
+    if (L.ready) {   // the inference operands depend on the activation
+        launch_mlp_pack(c->stream, L.master, L.pack, L.act);
+        int rc = check_launch(c, "mlp pack");
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    L.scale = 1.f;   // a different network: the judge recalibrates it
+    return FSL_OK;
+}
+
+int fsl_mlp_get_activation(const fsl_ctx* c) { return c->lrn.act; }
 
 int fsl_mlp_get_params(fsl_ctx* c, float* params, float* norm)
 {
@@ -337,7 +355,7 @@ int fsl_mlp_forward(fsl_ctx* c, const float* X, long long n, float* y)
     if (hipMemcpyAsync(dX, X, n * 8 * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         rc = set_err(c, FSL_ERR_HIP, "upload");
     if (!rc) {
-        launch_mlp_forward(c->stream, dX, n, c->lrn.pack, c->lrn.master, c->lrn.norm, dy);
+        launch_mlp_forward(c->stream, dX, n, c->lrn.pack, c->lrn.master, c->lrn.norm, dy, c->lrn.act);
         rc = check_launch(c, "mlp forward");
     }
     if (!rc && (hipMemcpyAsync(y, dy, n * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -570,9 +588,9 @@ int fsl_mlp_train(fsl_ctx* c, int epochs, long long batch, float lr, unsigned lo
         prof_mark(c, K_MLP_TRAIN, 0);
         for (long long s = 0; s < steps; s++) {
             launch_mlp_train_step(c->stream, L.X, L.t, L.n, batch, s, a, b, L.pack, L.master, L.norm, L.grad,
-                                  L.loss, L.wpow, L.winv);
+                                  L.loss, L.wpow, L.winv, L.act);
             launch_mlp_adam(c->stream, L.master, L.grad, L.adam_m, L.adam_v, lr, ++L.adam_t);
-            launch_mlp_pack(c->stream, L.master, L.pack);
+            launch_mlp_pack(c->stream, L.master, L.pack, L.act);
         }
         prof_mark(c, K_MLP_TRAIN, 1);
         if ((rc = check_launch(c, "training"))) return rc;
@@ -597,7 +615,7 @@ int fsl_mlp_gradient(fsl_ctx* c, long long batch, float* grad, float* loss)
     HIPCHK(c, hipMemsetAsync(L.loss, 0, sizeof(float), c->stream));
     // samples 0 .. batch-1 in order (permutation a = 1, b = 0)
     launch_mlp_train_step(c->stream, L.X, L.t, L.n, batch, 0, 1, 0, L.pack, L.master, L.norm, L.grad, L.loss,
-                          L.wpow, L.winv);
+                          L.wpow, L.winv, L.act);
     if ((rc = check_launch(c, "gradient"))) return rc;
     HIPCHK(c, hipMemcpyAsync(grad, L.grad, mlp_nparam() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     if (loss) HIPCHK(c, hipMemcpyAsync(loss, L.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -1,5 +1,5 @@
 // fsl_mlp.hip -- learned SGS closure: a per-cell MLP (8 -> 64 -> 64 -> 1,
-// tanh) that predicts the dynamic-Smagorinsky coefficient Cd = Cs^2 from
+// tanh or ReLU hidden layers) that predicts the dynamic-Smagorinsky coefficient Cd = Cs^2 from
 // normalised invariants of the resolved strain and rotation, a grid-scale
 // roughness ratio, |S| Delta and the distance from the layer centre,
 // replacing the test filter and the
@@ -120,6 +120,51 @@ __device__ __forceinline__ float act_r(float u)
     return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f);
 }
 
+// ReLU networks (ACT = FSL_MLP_RELU): h = max(z + b, 0).  The inference form
+// multiplies FP16 operands W1, W2, w3 onto accumulators that start at b1,
+// b2, b3, and takes the maximum after the conversion to FP16 (max and the
+// rounding commute): one v_pk_max_f16 per two units.
+constexpr int kTanh = 0, kRelu = 1;
+__device__ __forceinline__ h4 relu4(h4 v) { return __builtin_elementwise_max(v, (h4){0, 0, 0, 0}); }
+
+// FP16 bits of element e of v (through the packed 32-bit words)
+__device__ __forceinline__ unsigned short h4_bits(h4 v, int e)
+{
+    const u2 w = __builtin_bit_cast(u2, v);
+    return (unsigned short)((e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xffffu));
+}
+
+// activation of the training form (BF16 tiles, FP32 accumulator z, bias b)
+template <int ACT>
+__device__ __forceinline__ float act_train(float z, float b)
+{
+    if (ACT == kRelu) return fmaxf(z + b, 0.f);
+    return act(z, kTanhK * b);
+}
+
+// derivative of the activation from its value h
+template <int ACT>
+__device__ __forceinline__ float act_deriv(float h)
+{
+    if (ACT == kRelu) return h > 0.f ? 1.f : 0.f;
+    return 1.f - h * h;
+}
+
+// bias scale of the inference form's first-layer accumulator
+template <int ACT>
+__device__ __forceinline__ float infer_b1(float b)
+{
+    return ACT == kRelu ? b : kTanhK * b;
+}
+
+// four hidden values of the inference form from an output tile's accumulators
+template <int ACT>
+__device__ __forceinline__ h4 act4(const f4 z)
+{
+    if (ACT == kRelu) return relu4(cvt4h(z[0], z[1], z[2], z[3]));
+    return cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
+}
+
 // Each wavefront of the inference kernel stages its own 64 cells' features in
 // its own LDS slice and reads them back across its lanes: ordering the
 // wave's LDS accesses suffices (s_waitcnt lgkmcnt(0)); no workgroup barrier,
@@ -154,10 +199,11 @@ __device__ __forceinline__ bf4 lds_row4(const unsigned short* p)
 }
 
 // Forward pass of 64 samples held in ws.X (already normalised; BF16, FP16
-// for INFER); leaves H1, H2 in LDS (BF16 h, FP16 r for INFER) and the
-// outputs in ws.y.  Training (INFER false) uses the BF16 tiles and tanh; the
-// inference form matches k_mlp_mu_tr's arithmetic.  Caller syncs.
-template <bool INFER>
+// for INFER); leaves H1, H2 in LDS (BF16 h; for INFER FP16 r of tanh
+// networks, FP16 h of ReLU networks) and the outputs in ws.y.  Training
+// (INFER false) uses the BF16 tiles; the inference form matches
+// k_mlp_mu_tr's arithmetic.  Caller syncs.
+template <bool INFER, int ACT>
 __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
 {
     const int r16 = lane & 15, q4 = lane >> 4;
@@ -166,13 +212,15 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
     for (int tm = 0; tm < 4; tm++) {
         const bf4 a = lds_row4(&ws.X[(tm * 16 + r16) * KP1 + 4 * q4]);
         for (int tn = 0; tn < 4; tn++) {
-            const float kb = kTanhK * w.master[OB1 + tn * 16 + r16];
+            const float b = w.master[OB1 + tn * 16 + r16], kb = infer_b1<ACT>(b);
             f4 acc;
             if (INFER) acc = mfma_h(as_h4(a), as_h4(w.pack[PH1 + tn * 64 + lane]), f4{kb, kb, kb, kb});
             else acc = mfma(a, w.pack[PB1 + tn * 64 + lane], f4{0.f, 0.f, 0.f, 0.f});
+            const h4 hi = act4<ACT>(acc);
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] = INFER ? f2h(act_r(acc[e])) : f2bf(act(acc[e], kb));
+                ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
+                    INFER ? h4_bits(hi, e) : f2bf(act_train<ACT>(acc[e], b));
         }
     }
     __syncthreads();
@@ -196,11 +244,12 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
             }
         }
         for (int tn = 0; tn < 4; tn++) {
-            const float kb = kTanhK * w.master[OB2 + tn * 16 + r16];
+            const float b = w.master[OB2 + tn * 16 + r16];
+            const h4 hi = act4<ACT>(acc[tn]);
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
-                    INFER ? f2h(act_r(acc[tn][e])) : f2bf(act(acc[tn][e], kb));
+                    INFER ? h4_bits(hi, e) : f2bf(act_train<ACT>(acc[tn][e], b));
         }
     }
     __syncthreads();
@@ -466,7 +515,7 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
 // pass through LDS (one 8-byte read per lane and 16-cell tile).
 constexpr int MU_CELLS = 256;   // cells per block iteration (4 waves x 64)
 
-template <int TP>   // 16-cell tiles in flight per layer
+template <int TP, int ACT>   // 16-cell tiles in flight per layer; hidden activation
 __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double* mu, float* cd_out, float scale)
 {
     __shared__ unsigned short sX[4][64 * NIN];
@@ -483,7 +532,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         A2[1][t] = pack8(w.pack + PH2, 1, t, 4, lane);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            kb1[t][e] = kTanhK * w.master[OB1 + t * 16 + 4 * q4 + e];
+            kb1[t][e] = infer_b1<ACT>(w.master[OB1 + t * 16 + 4 * q4 + e]);
             kb2[t][e] = cf[t * 16 + 4 * q4 + e];
         }
     }
@@ -564,7 +613,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
                     const f4 z = mfma_h(A1[t], xb[u], f4{kb1[t][0], kb1[t][1], kb1[t][2], kb1[t][3]});
-                    h1[u][t] = cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
+                    h1[u][t] = act4<ACT>(z);
                 }
 #pragma unroll
             for (int t = 0; t < 4; t++)
@@ -573,7 +622,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
                     f4 z = {kb2[t][0], kb2[t][1], kb2[t][2], kb2[t][3]};
                     z = mfma_h32(A2[0][t], cat8(h1[u][0], h1[u][1]), z);
                     z = mfma_h32(A2[1][t], cat8(h1[u][2], h1[u][3]), z);
-                    h2[u][t] = cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
+                    h2[u][t] = act4<ACT>(z);
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
@@ -607,6 +656,7 @@ int mu_variant()
 }
 
 // forward pass on caller-given raw features (normalised as in k_mlp_mu_tr)
+template <int ACT>
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, MlpW w, float* y)
 {
     __shared__ WaveLds lds[4];
@@ -618,7 +668,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
-    forward64<true>(ws, w, lane);
+    forward64<true, ACT>(ws, w, lane);
     if (s < n) y[s] = ws.y[lane];
 }
 
@@ -635,6 +685,7 @@ struct TrainArgs {
     float wpow, winv;    // sample weight (|S| Delta / dU)^wpow * winv (feature 6); wpow 0: 1
 };
 
+template <int ACT>
 __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
 {
     __shared__ WaveLds lds[4];
@@ -651,14 +702,14 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
-    forward64<false>(ws, w, lane);
+    forward64<false, ACT>(ws, w, lane);
     const float wgt = T.wpow != 0.f ? powf(fmaxf(T.X[s * NIN + 6], 0.f), T.wpow) * T.winv : 1.f;
     const float err = ws.y[lane] - T.t[s];
     const float dy = wgt * err * (2.f / (float)T.batch);
     sdy[wv][lane] = dy;
     atomicAdd(T.loss, wgt * err * err);
     __syncthreads();
-    // layer 3: grad w3[k] = sum_s H2[s][k] dy[s], grad b3; dZ2 = dy w3 (1 - H2^2)
+    // layer 3: grad w3[k] = sum_s H2[s][k] dy[s], grad b3; dZ2 = dy w3 act'(H2)
     {
         float gw = 0.f, gb = 0.f;
         for (int q = 0; q < 64; q++) {
@@ -669,7 +720,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
         if (lane == 0) atomicAdd(&T.grad[OB3], gb);
         for (int kk = 0; kk < NH; kk++) {
             const float h = bf2f(ws.H2[lane * NH + kk]);
-            dz[lane * NH + kk] = f2bf(dy * w.master[OW3 + kk] * (1.f - h * h));
+            dz[lane * NH + kk] = f2bf(dy * w.master[OW3 + kk] * act_deriv<ACT>(h));
         }
     }
     __syncthreads();
@@ -699,7 +750,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
                     atomicAdd(&T.grad[OW2 + (tj * 16 + 4 * q4 + e) * NH + tk * 16 + r16], acc[e]);
             }
     }
-    // dZ1 = (dZ2 W2^T) (1 - H1^2) -> H2 buffer
+    // dZ1 = (dZ2 W2^T) act'(H1) -> H2 buffer
     {
         f4 acc[4][4];
         for (int tm = 0; tm < 4; tm++)
@@ -716,7 +767,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(TrainArgs T, MlpW w)
                 for (int e = 0; e < 4; e++) {
                     const int sr = tm * 16 + 4 * q4 + e, jc = tn * 16 + r16;
                     const float h = bf2f(ws.H1[sr * NH + jc]);
-                    ws.H2[sr * NH + jc] = f2bf(acc[tm][tn][e] * (1.f - h * h));
+                    ws.H2[sr * NH + jc] = f2bf(acc[tm][tn][e] * act_deriv<ACT>(h));
                 }
     }
     __syncthreads();
@@ -771,24 +822,29 @@ __global__ void k_mlp_adam(AdamArgs a)
     a.grad[p] = 0.f;
 }
 
-// BF16 B operands in lane order from the FP32 master weights
-__global__ void k_mlp_pack(const float* master, bf4* pack)
+// BF16 B operands in lane order from the FP32 master weights, and the FP16
+// operands and FP32 constants of the inference form of activation act
+__global__ void k_mlp_pack(const float* master, bf4* pack, int act)
 {
+    const bool relu = act == kRelu;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // one (tile, lane) per thread
     if (t >= NPACK) return;
     if (t >= PHF) {
         // FP32 constants of the inference form, two per 8-byte slot:
         // K (b2[n] + sum_k W2[k][n]) for n < 64, b3 + sum_k w3[k] at 64
+        // (tanh); b2[n], b3 (ReLU)
         float c[2] = {0.f, 0.f};
         for (int e = 0; e < 2; e++) {
             const int n = 2 * (t - PHF) + e;
             if (n < NH) {
                 float sum = master[OB2 + n];
-                for (int k2 = 0; k2 < NH; k2++) sum += master[OW2 + k2 * NH + n];
-                c[e] = kTanhK * sum;
+                if (!relu)
+                    for (int k2 = 0; k2 < NH; k2++) sum += master[OW2 + k2 * NH + n];
+                c[e] = relu ? sum : kTanhK * sum;
             } else if (n == NH) {
                 float sum = master[OB3];
-                for (int k2 = 0; k2 < NH; k2++) sum += master[OW3 + k2];
+                if (!relu)
+                    for (int k2 = 0; k2 < NH; k2++) sum += master[OW3 + k2];
                 c[e] = sum;
             }
         }
@@ -813,19 +869,19 @@ __global__ void k_mlp_pack(const float* master, bf4* pack)
         } else if (tile < PH1 / 64) {               // w3 in column 0
             const int ks = tile - PB3 / 64;
             x = r16 == 0 ? master[OW3 + ks * 16 + kk] : 0.f;
-        } else if (tile < PH2 / 64) {               // K W1 (FP16)
+        } else if (tile < PH2 / 64) {               // K W1 (FP16; W1 for ReLU)
             const int n = (tile - PH1 / 64) * 16 + r16;
-            x = kk < NIN ? kTanhK * master[OW1 + kk * NH + n] : 0.f;
-        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16), 16x16x32 order
+            x = kk < NIN ? (relu ? 1.f : kTanhK) * master[OW1 + kk * NH + n] : 0.f;
+        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16; W2), 16x16x32 order
             const int u = t - PH2, pos = u >> 1, half = u & 1;
             const int sl = pos / 256, tn = (pos / 64) % 4, ln = pos % 64;
             const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e, nn = tn * 16 + (ln & 15);
-            x = -2.f * kTanhK * master[OW2 + (ks * 16 + kq) * NH + nn];
-        } else {                                    // -2 w3 in column 0 (FP16), 16x16x32 order
+            x = (relu ? 1.f : -2.f * kTanhK) * master[OW2 + (ks * 16 + kq) * NH + nn];
+        } else {                                    // -2 w3 in column 0 (FP16; w3), 16x16x32 order
             const int u = t - PH3, pos = u >> 1, half = u & 1;
             const int sl = pos / 64, ln = pos % 64;
             const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e;
-            x = (ln & 15) == 0 ? -2.f * master[OW3 + ks * 16 + kq] : 0.f;
+            x = (ln & 15) == 0 ? (relu ? 1.f : -2.f) * master[OW3 + ks * 16 + kq] : 0.f;
         }
         v[e] = x;
     }
@@ -977,9 +1033,9 @@ __global__ void __launch_bounds__(256) k_diag_dw(const double* zs, Geo g, double
 int mlp_nparam() { return NPARAM; }
 int mlp_npack() { return NPACK; }
 
-void launch_mlp_pack(hipStream_t s, const float* master, void* pack)
+void launch_mlp_pack(hipStream_t s, const float* master, void* pack, int act)
 {
-    hipLaunchKernelGGL(k_mlp_pack, dim3((NPACK + 255) / 256), dim3(256), 0, s, master, (bf4*)pack);
+    hipLaunchKernelGGL(k_mlp_pack, dim3((NPACK + 255) / 256), dim3(256), 0, s, master, (bf4*)pack, act);
 }
 
 static MlpW mlpw(const void* pack, const float* master, const float* norm)
@@ -992,15 +1048,16 @@ static MlpW mlpw(const void* pack, const float* master, const float* norm)
 }
 
 void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* pack, const float* master,
-                        const float* norm, float* y)
+                        const float* norm, float* y, int act)
 {
-    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, n,
-                       mlpw(pack, master, norm), y);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (act == kRelu) hipLaunchKernelGGL(k_mlp_forward<kRelu>, grid, dim3(256), 0, s, X, n, mlpw(pack, master, norm), y);
+    else hipLaunchKernelGGL(k_mlp_forward<kTanh>, grid, dim3(256), 0, s, X, n, mlpw(pack, master, norm), y);
 }
 
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                    double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out,
-                   float scale)
+                   float scale, int act)
 {
     FeatArgs a;
     a.U = U;
@@ -1024,9 +1081,15 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     // cells; a multiple of the 8 XCDs (the kernel's order assumes it)
     const dim3 grid((unsigned)std::max<long long>(8, std::min<long long>(nblk, 3LL * ncu) / 8 * 8));
     const MlpW w = mlpw(pack, master, norm);
-    if (tp == 1) hipLaunchKernelGGL(k_mlp_mu_tr<1>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-    else if (tp == 4) hipLaunchKernelGGL(k_mlp_mu_tr<4>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-    else hipLaunchKernelGGL(k_mlp_mu_tr<2>, grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+    if (act == kRelu) {
+        if (tp == 1) hipLaunchKernelGGL((k_mlp_mu_tr<1, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        else if (tp == 4) hipLaunchKernelGGL((k_mlp_mu_tr<4, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        else hipLaunchKernelGGL((k_mlp_mu_tr<2, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+    } else {
+        if (tp == 1) hipLaunchKernelGGL((k_mlp_mu_tr<1, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        else if (tp == 4) hipLaunchKernelGGL((k_mlp_mu_tr<4, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        else hipLaunchKernelGGL((k_mlp_mu_tr<2, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+    }
 }
 
 void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
@@ -1048,7 +1111,7 @@ void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph,
 void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long long n, long long batch,
                            long long step, unsigned long long pa, unsigned long long pc, const void* pack,
                            const float* master, const float* norm, float* grad, float* loss, float wpow,
-                           float winv)
+                           float winv, int act)
 {
     TrainArgs T;
     T.wpow = wpow;
@@ -1062,7 +1125,9 @@ void launch_mlp_train_step(hipStream_t s, const float* X, const float* t, long l
     T.perm_c = pc;
     T.grad = grad;
     T.loss = loss;
-    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)(batch / 256)), dim3(256), 0, s, T, mlpw(pack, master, norm));
+    const dim3 grid((unsigned)(batch / 256));
+    if (act == kRelu) hipLaunchKernelGGL(k_mlp_grad<kRelu>, grid, dim3(256), 0, s, T, mlpw(pack, master, norm));
+    else hipLaunchKernelGGL(k_mlp_grad<kTanh>, grid, dim3(256), 0, s, T, mlpw(pack, master, norm));
 }
 
 void launch_mlp_adam(hipStream_t s, float* master, float* grad, float* m, float* v, float lr, int t)

@@ -20,7 +20,8 @@
  * --no-output skips probes.dat and the frames.
  *
  * --learned MODEL runs the learned SGS closure (parameter file as written by
- * fsl_mlp_file_write), judged against the dynamic model every 50 steps.
+ * fsl_mlp_file_write or fsl_mlp_file_write_act, with its activation),
+ * judged against the dynamic model every 50 steps.
  */
 #include "fsl.h"
 
@@ -124,7 +125,9 @@ int main(int argc, char **argv)
     if (model) {
         static float params[FSL_MLP_NPARAM];
         float norm[16];
-        if (fsl_mlp_file_read(model, params, norm) != FSL_OK || fsl_mlp_set_params(ctx, params, norm) != FSL_OK) {
+        int act = FSL_MLP_TANH;
+        if (fsl_mlp_file_read_act(model, params, norm, &act) != FSL_OK || fsl_mlp_set_params(ctx, params, norm) != FSL_OK ||
+            fsl_mlp_set_activation(ctx, act) != FSL_OK) {
             fprintf(stderr, "cannot load the learned closure from %s\n", model);
             fsl_destroy(ctx);
             free(U);

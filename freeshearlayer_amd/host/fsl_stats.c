@@ -37,29 +37,43 @@ void fsl_mlp_init_params(unsigned long long seed, float *params, float *norm)
 }
 
 static const char kMagic[8] = {'F', 'S', 'L', 'M', 'L', 'P', '0', '1'};
+static const char kMagic2[8] = {'F', 'S', 'L', 'M', 'L', 'P', '0', '2'};
 
-int fsl_mlp_file_write(const char *path, const float *params, const float *norm)
+int fsl_mlp_file_write_act(const char *path, const float *params, const float *norm, int activation)
 {
-    const int hdr[3] = {FSL_MLP_NIN, FSL_MLP_NHID, FSL_MLP_NPARAM};
-    FILE *f = fopen(path, "wb");
+    const int hdr[4] = {FSL_MLP_NIN, FSL_MLP_NHID, FSL_MLP_NPARAM, activation};
+    const int v2 = activation != FSL_MLP_TANH;
+    FILE *f;
     int ok;
+    if (activation != FSL_MLP_TANH && activation != FSL_MLP_RELU) return FSL_ERR_ARG;
+    f = fopen(path, "wb");
     if (!f) return FSL_ERR_IO;
-    ok = fwrite(kMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(int), 3, f) == 3 &&
+    ok = fwrite(v2 ? kMagic2 : kMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(int), v2 ? 4 : 3, f) == (size_t)(v2 ? 4 : 3) &&
          fwrite(norm, sizeof(float), 16, f) == 16 &&
          fwrite(params, sizeof(float), FSL_MLP_NPARAM, f) == FSL_MLP_NPARAM;
     if (fclose(f) != 0) ok = 0;
     return ok ? FSL_OK : FSL_ERR_IO;
 }
 
-int fsl_mlp_file_read(const char *path, float *params, float *norm)
+int fsl_mlp_file_write(const char *path, const float *params, const float *norm)
+{
+    return fsl_mlp_file_write_act(path, params, norm, FSL_MLP_TANH);
+}
+
+int fsl_mlp_file_read_act(const char *path, float *params, float *norm, int *activation)
 {
     char m[8];
-    int hdr[3], i;
+    int hdr[4] = {0, 0, 0, FSL_MLP_TANH}, i, v2;
     FILE *f = fopen(path, "rb");
     if (!f) return FSL_ERR_IO;
-    if (fread(m, 1, 8, f) != 8 || memcmp(m, kMagic, 8) != 0 || fread(hdr, sizeof(int), 3, f) != 3 ||
-        hdr[0] != FSL_MLP_NIN || hdr[1] != FSL_MLP_NHID || hdr[2] != FSL_MLP_NPARAM ||
-        fread(norm, sizeof(float), 16, f) != 16 ||
+    if (fread(m, 1, 8, f) != 8 || (memcmp(m, kMagic, 8) != 0 && memcmp(m, kMagic2, 8) != 0)) {
+        fclose(f);
+        return FSL_ERR_IO;
+    }
+    v2 = memcmp(m, kMagic2, 8) == 0;
+    if (fread(hdr, sizeof(int), v2 ? 4 : 3, f) != (size_t)(v2 ? 4 : 3) || hdr[0] != FSL_MLP_NIN ||
+        hdr[1] != FSL_MLP_NHID || hdr[2] != FSL_MLP_NPARAM ||
+        (hdr[3] != FSL_MLP_TANH && hdr[3] != FSL_MLP_RELU) || fread(norm, sizeof(float), 16, f) != 16 ||
         fread(params, sizeof(float), FSL_MLP_NPARAM, f) != FSL_MLP_NPARAM) {
         fclose(f);
         return FSL_ERR_IO;
@@ -69,7 +83,16 @@ int fsl_mlp_file_read(const char *path, float *params, float *norm)
         if (!isfinite(norm[i])) return FSL_ERR_NONFINITE;
     for (i = 0; i < FSL_MLP_NPARAM; i++)
         if (!isfinite(params[i])) return FSL_ERR_NONFINITE;
+    if (activation) *activation = hdr[3];
     return FSL_OK;
+}
+
+int fsl_mlp_file_read(const char *path, float *params, float *norm)
+{
+    int act = FSL_MLP_TANH;
+    const int rc = fsl_mlp_file_read_act(path, params, norm, &act);
+    if (rc != FSL_OK) return rc;
+    return act == FSL_MLP_TANH ? FSL_OK : FSL_ERR_UNSUPPORTED;
 }
 
 int fsl_shear_statistics(const fsl_case *c, const double *prof, double U1, double U2, double rho0,

@@ -61,8 +61,7 @@ def run(path, mesh, args, model):
     ctx, case, U, u_in, shear = shear_case.make(L, H, D, sgs=sgs, precision=prec, mach=args.mach,
                                                 reynolds=args.reynolds, pert=args.pert, seed=args.seed)
     if sgs == "learned":
-        p, n = fsl.mlp_file_read(model)
-        ctx.mlp_set_params(p, n)
+        ctx.mlp_load(model)
         ctx.learned_configure(args.interval, args.tol)
     rec = {"path": path, "sgs": sgs, "precision": prec, "step": [], "delta_theta": [], "delta_theta_out": [],
            "delta_w": [], "tke": [], "mass": [], "energy": [], "dt": []}

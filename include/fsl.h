@@ -326,8 +326,9 @@ int fsl_favre_average(const char *path, unsigned HIG, unsigned step_start, unsig
 int fsl_favre_write_profiles(const char *dir, unsigned HIG, const double *ru);
 
 /* --- learned SGS closure (sgs_model = FSL_SGS_LEARNED) --- */
-/* Network 8 -> 64 -> 64 -> 1 with tanh hidden layers; BF16 operands on the
-   matrix cores with FP32 accumulation.  Output y gives Cd = clip(0.15 y, 0,
+/* Network 8 -> 64 -> 64 -> 1 with tanh or ReLU hidden layers
+   (fsl_mlp_set_activation); BF16 operands on the matrix cores with FP32
+   accumulation for training, FP16 operands for inference.  Output y gives Cd = clip(0.15 y, 0,
    0.15) and mu_SGS = rho Cd Delta^2 |S| (src/turbulence.c:436-445 with the
    network's Cd in place of the Germano-Lilly one).  Features per interior
    cell, from central differences of u = (rho u)/rho (FP32):
@@ -346,10 +347,25 @@ int fsl_favre_write_profiles(const char *dir, unsigned HIG, const double *ru);
 #define FSL_MLP_NPARAM 4801
 /* host: Glorot-uniform weights (splitmix64 of seed), zero biases, identity norm */
 void fsl_mlp_init_params(unsigned long long seed, float *params, float *norm);
-/* parameter file: "FSLMLP01", int32 nin, nhid, nparam, float norm[16], float params[nparam] */
+/* Hidden-layer activation: tanh (the default) or ReLU h = max(z, 0).  The
+   inference of a ReLU network takes one packed FP16 maximum per two hidden
+   units where tanh takes v_exp_f32, v_rcp_f32 and an add per unit. */
+#define FSL_MLP_TANH 0
+#define FSL_MLP_RELU 1
+/* parameter file: "FSLMLP01", int32 nin, nhid, nparam, float norm[16], float
+   params[nparam] (tanh); "FSLMLP02": int32 nin, nhid, nparam, activation, then
+   the same.  fsl_mlp_file_write writes FSLMLP01; fsl_mlp_file_read reads
+   either and refuses (FSL_ERR_UNSUPPORTED) a file whose activation is not
+   tanh; the _act forms carry the activation. */
 int fsl_mlp_file_write(const char *path, const float *params, const float *norm);
 int fsl_mlp_file_read(const char *path, float *params, float *norm);
+int fsl_mlp_file_write_act(const char *path, const float *params, const float *norm, int activation);
+int fsl_mlp_file_read_act(const char *path, float *params, float *norm, int *activation);
 int fsl_mlp_set_params(fsl_ctx *ctx, const float *params, const float *norm);
+/* the context network's activation (default FSL_MLP_TANH); training and
+   inference both use it */
+int fsl_mlp_set_activation(fsl_ctx *ctx, int activation);
+int fsl_mlp_get_activation(const fsl_ctx *ctx);
 int fsl_mlp_get_params(fsl_ctx *ctx, float *params, float *norm);
 /* network output for n host rows of 8 raw features (normalised with norm) */
 int fsl_mlp_forward(fsl_ctx *ctx, const float *X, long long n, float *y);
@@ -388,19 +404,21 @@ int fsl_mlp_train(fsl_ctx *ctx, int epochs, long long batch, float lr, unsigned 
    loss receives the sum of weighted squared errors */
 int fsl_mlp_gradient(fsl_ctx *ctx, long long batch, float *grad, float *loss);
 /* Judge: every interval steps (steps 1, 1 + interval, ...) both models run
-   and two errors of the network's mu_SGS against the dynamic model's are
-   formed: the relative difference of the SGS dissipation D = sum(mu_SGS
-   |S|^2) over the interior, and the profile error || s P_l - P_d ||_2 /
-   || P_d ||_2 of the y profiles P(j) of the dissipation (x-z sums of each
-   row) with s = D_d / D_l, which measures its distribution across the layer
-   once the level is matched.  When the first exceeds tol or the second
-   profile_tol (fsl_learned_set_profile_tol, default 0.25) the dynamic model
-   is used, and kept every step until a later judge passes.  With adapt on, each judge also sets the network's
-   output scale (Cd = clip(0.15 scale y, 0, 0.15)) to scale D_dynamic /
-   D_learned, clamped to [0.1, 10], for the following steps: the network
-   gives the spatial distribution of Cd, the dynamic procedure its global
-   level.  Defaults: interval 50, tol 0.10, scale 1, adapt on; interval 0
-   never judges. */
+   on the step state and three errors of the network's mu_SGS against the
+   dynamic model's are formed, with e = mu_SGS |S|^2 the SGS dissipation
+   density and D its sum over the interior: the level error |D_l - D_d| /
+   D_d; the profile error || s P_l - P_d ||_2 / || P_d ||_2 of the y
+   profiles P(j) (x-z sums of e in each row) and the local error
+   || s e_l - e_d ||_2 / || e_d ||_2 over the cells, both with s = D_d / D_l
+   (the distribution once the level is matched).  When the level error
+   exceeds tol, or the profile or local error its bound
+   (fsl_learned_set_tolerances; default 1e30: reported, not enforced), the
+   step runs on the dynamic model, and so does every step until a later
+   judge passes.  With adapt on, each judge also sets the network's output
+   scale (Cd = clip(0.15 scale y, 0, 0.15)) to scale D_d / D_l, clamped to
+   [0.1, 10], for the following steps: the network gives the spatial
+   distribution of Cd, the dynamic procedure its global level.  Defaults:
+   interval 50, tol 0.10, scale 1, adapt on; interval 0 never judges. */
 int fsl_learned_configure(fsl_ctx *ctx, int interval, double tol);
 int fsl_learned_set_tolerances(fsl_ctx *ctx, double profile_tol, double local_tol);
 int fsl_learned_set_scale(fsl_ctx *ctx, double scale, int adapt);

@@ -1,5 +1,5 @@
 """numpy restatement of the learned SGS closure (test infrastructure only):
-BF16 rounding, the 8 -> 64 -> 64 -> 1 tanh network with BF16 operands and
+BF16 rounding, the 8 -> 64 -> 64 -> 1 network (tanh or ReLU hidden layers) with BF16 operands and
 FP32 accumulation, its mean-squared-error gradient in the same rounding
 steps as the device kernel, the per-cell features and the plane profiles.
 The parameter layout is the one include/fsl.h documents."""
@@ -38,16 +38,32 @@ def f16(x):
     return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
 
 
-def forward(X, params, norm, exact=False, inference=False):
+def forward(X, params, norm, exact=False, inference=False, act="tanh"):
     """Network output for raw features X (n x 8).  exact=True skips every
     BF16 rounding (FP32 network).  inference=True follows the inference
     kernels instead: FP16 operands, features clamped to +-60000, hidden values
     r = 1 / (2^u + 1) of the scaled arguments u (tanh = 1 - 2 r): layer 1
     FP16(x) FP16(K W1) + K b1, layer 2 FP16(r1) FP16(-2 K W2) +
     K (b2 + sum_k W2[k]), output FP16(r2) FP16(-2 w3) + b3 + sum_k w3[k];
-    training uses BF16(x), BF16(W) and tanh."""
+    training uses BF16(x), BF16(W) and tanh.  act="relu": h = max(z, 0),
+    inference with FP16(x) FP16(W1) + b1, FP16(h1) FP16(W2) + b2 and
+    FP16(h2) FP16(w3) + b3, the hidden values rounded to FP16."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
     xn = (np.asarray(X, np.float32) - norm[:8]) * norm[8:]
+    if act == "relu":
+        relu = lambda z: np.maximum(z, 0.0)
+        if inference and not exact:
+            x = f16(np.clip(xn, -60000.0, 60000.0))
+            h1 = f16(relu(x.astype(np.float64) @ f16(W1) + b1))
+            h2 = f16(relu(h1.astype(np.float64) @ f16(W2) + b2))
+            y = h2.astype(np.float64) @ f16(w3) + b3
+            return y.astype(np.float32), (x, h1, h2)
+        r = (lambda a: np.asarray(a, np.float32)) if exact else bf16
+        x = r(xn)
+        h1 = r(relu(x.astype(np.float64) @ r(W1) + b1).astype(np.float32))
+        h2 = r(relu(h1.astype(np.float64) @ r(W2) + b2).astype(np.float32))
+        y = h2.astype(np.float64) @ r(w3) + b3
+        return y.astype(np.float32), (x, h1, h2)
     if inference and not exact:
         x = f16(np.clip(xn, -60000.0, 60000.0))
         u1 = x.astype(np.float64) @ f16(TANH_K * W1) + TANH_K * b1
@@ -65,11 +81,15 @@ def forward(X, params, norm, exact=False, inference=False):
     return y.astype(np.float32), (x, h1, h2)
 
 
-def gradient(X, t, params, norm, weight=None):
+def gradient(X, t, params, norm, weight=None, act="tanh"):
     """Gradient of mean(w (y - t)^2) with the device kernel's rounding steps:
     BF16 activations and back-propagated errors, FP32 master w3."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
-    y, (x, h1, h2) = forward(X, params, norm)
+    y, (x, h1, h2) = forward(X, params, norm, act=act)
+    if act == "relu":
+        d2, d1 = (h2 > 0).astype(np.float64), (h1 > 0).astype(np.float64)
+    else:
+        d2, d1 = 1.0 - h2.astype(np.float64) ** 2, 1.0 - h1.astype(np.float64) ** 2
     n = len(t)
     err = (y - t).astype(np.float64)
     w = np.ones(n) if weight is None else np.asarray(weight, np.float64)
@@ -77,11 +97,10 @@ def gradient(X, t, params, norm, weight=None):
     g = np.zeros(NPARAM)
     g[OW3:OB3] = h2.T.astype(np.float64) @ dy
     g[OB3] = dy.sum()
-    dz2 = bf16((dy[:, None] * w3[None, :] * (1.0 - h2.astype(np.float64) ** 2)).astype(np.float32))
+    dz2 = bf16((dy[:, None] * w3[None, :] * d2).astype(np.float32))
     g[OB2:OW3] = dz2.sum(0)
     g[OW2:OB2] = (h1.T.astype(np.float64) @ dz2).reshape(-1)
-    dz1 = bf16(((dz2.astype(np.float64) @ bf16(W2).T.astype(np.float64)) *
-                (1.0 - h1.astype(np.float64) ** 2)).astype(np.float32))
+    dz1 = bf16(((dz2.astype(np.float64) @ bf16(W2).T.astype(np.float64)) * d1).astype(np.float32))
     g[OB1:OW2] = dz1.sum(0)
     g[OW1:OB1] = (x.T.astype(np.float64) @ dz1).reshape(-1)
     return g, float((w * err ** 2).sum())

@@ -64,8 +64,7 @@ def test_mixed_tracks_fp64_at_384x256x256(mach):
 def spun_up_learned(mesh, mach, reynolds=500.0, pert=0.02, cap=300):
     ctx, case, U0, u_in, _ = shear_case.make(*mesh, sgs="learned", precision="mixed", fused=True, cfl=0.5,
                                              mach=mach, reynolds=reynolds, pert=pert)
-    p, n = fsl.mlp_file_read(MODEL)
-    ctx.mlp_set_params(p, n)
+    ctx.mlp_load(MODEL)
     ctx.learned_configure(50, 0.10)
     steps = 0
     while True:

@@ -13,14 +13,16 @@ from freeshearlayer_amd import study, train_sgs
 pytestmark = pytest.mark.gpu
 
 
-def test_train_sgs_small(tmp_path):
+@pytest.mark.parametrize("act", ["tanh", "relu"])
+def test_train_sgs_small(tmp_path, act):
     out, rep = str(tmp_path / "m.bin"), str(tmp_path / "r.json")
-    rc = train_sgs.main(["--out", out, "--report", rep, "--mesh", "24x20x16", "--mach", "0.3,0.5",
-                         "--reynolds", "500", "--samples", "60000", "--steps", "60", "--every", "10",
+    rc = train_sgs.main(["--out", out, "--report", rep, "--mesh", "24x20x16", "--runs", "0.3:500:0.02,0.5:500:0.02",
+                         "--samples", "60000", "--steps", "60", "--every", "10",
                          "--batch", "2048", "--schedule", "2:2e-3", "--weight-power", "0,3", "--seeds", "1",
-                         "--eval", "0.4:500", "--eval-steps", "100"])
+                         "--eval", "0.4:500:0.02", "--eval-steps", "100", "--activation", act])
     assert rc == 0
-    p, n = fsl.mlp_file_read(out)
+    p, n, a = fsl.mlp_file_read_act(out)
+    assert a == fsl.MLP_ACTIVATIONS[act]
     assert p.shape == (fsl.MLP_NPARAM,) and np.all(np.isfinite(p)) and np.all(n[8:] > 0)
     r = json.load(open(rep))
     assert len(r["candidates"]) == 2 and r["selected"]["weight_power"] in (0.0, 3.0)

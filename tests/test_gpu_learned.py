@@ -45,7 +45,8 @@ def test_mfma_operand_layout():
     np.testing.assert_allclose(D, ref, rtol=1e-5, atol=1e-5)
 
 
-def test_forward_matches_numpy():
+@pytest.mark.parametrize("act", ["tanh", "relu"])
+def test_forward_matches_numpy(act):
     rng = np.random.default_rng(2)
     p, _ = fsl.mlp_init_params(11)
     p[mlp_ref.OB1:mlp_ref.OW2] = rng.uniform(-0.3, 0.3, 64)
@@ -55,11 +56,13 @@ def test_forward_matches_numpy():
     ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
     with ctx:
         ctx.mlp_set_params(p, norm)
+        ctx.mlp_set_activation(act)
         y = ctx.mlp_forward(X)
         q, m = ctx.mlp_get_params()
+        assert ctx.mlp_activation() == fsl.MLP_ACTIVATIONS[act]
     assert np.array_equal(q, p) and np.array_equal(m, norm)
-    yr, _ = mlp_ref.forward(X, p, norm, inference=True)
-    ye, _ = mlp_ref.forward(X, p, norm, exact=True)
+    yr, _ = mlp_ref.forward(X, p, norm, inference=True, act=act)
+    ye, _ = mlp_ref.forward(X, p, norm, exact=True, act=act)
     d = np.abs(y - yr)
     print("forward: max |y - y_bf16| %.2e, mean %.2e, max |y - y_fp32| %.2e"
           % (d.max(), d.mean(), np.abs(y - ye).max()))
@@ -67,7 +70,8 @@ def test_forward_matches_numpy():
     assert np.abs(y - ye).max() < 0.1 * max(1.0, np.abs(ye).max())
 
 
-def test_gradient_matches_numpy():
+@pytest.mark.parametrize("act", ["tanh", "relu"])
+def test_gradient_matches_numpy(act):
     rng = np.random.default_rng(3)
     p, norm = fsl.mlp_init_params(5)
     X = rng.standard_normal((1024, 8)).astype(np.float32)
@@ -75,9 +79,10 @@ def test_gradient_matches_numpy():
     ctx, *_ = shear_ctx(8, 8, 8, sgs=fsl.FSL_SGS_DYNAMIC)
     with ctx:
         ctx.mlp_set_params(p, norm)
+        ctx.mlp_set_activation(act)
         ctx.mlp_load_samples(X, t)
         g, loss = ctx.mlp_gradient(1024)
-    gr, lr = mlp_ref.gradient(X, t, p, norm)
+    gr, lr = mlp_ref.gradient(X, t, p, norm, act=act)
     assert abs(loss - lr) <= 2e-2 * lr
     groups = {"W1": (0, 512), "b1": (512, 576), "W2": (576, 4672), "b2": (4672, 4736), "w3": (4736, 4800),
               "b3": (4800, 4801)}
@@ -153,7 +158,8 @@ def test_plane_profiles_and_thickness():
     np.testing.assert_allclose(dw, ref, rtol=1e-6)
 
 
-def test_features_and_learned_mu():
+@pytest.mark.parametrize("act", ["tanh", "relu"])
+def test_features_and_learned_mu(act):
     L, H, D = 20, 16, 12
     case = fsl.case_reference(L, H, D, BL_HIG=5)
     U = random_state(L, H, D, seed=21)
@@ -170,6 +176,7 @@ def test_features_and_learned_mu():
         p, _ = fsl.mlp_init_params(13)
         norm = np.concatenate([Xg.mean(0), 1 / Xg.std(0)]).astype(np.float32)
         ctx.mlp_set_params(p, norm)
+        ctx.mlp_set_activation(act)
         ctx.learned_smagorinsky()
         mu = ctx.download_mu()
         y = ctx.mlp_forward(Xg)

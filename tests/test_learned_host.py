@@ -47,6 +47,34 @@ def test_param_file_round_trip_and_rejects(tmp_path):
         fsl.mlp_file_read(bad)
 
 
+def test_param_file_activation(tmp_path):
+    """FSLMLP02 files carry the activation; the FSLMLP01 reader refuses a
+    non-tanh one, the _act reader takes either format."""
+    p, n = fsl.mlp_init_params(2)
+    relu = str(tmp_path / "relu.bin")
+    fsl.mlp_file_write_act(relu, p, n, "relu")
+    assert os.path.getsize(relu) == 8 + 16 + 64 + 4 * fsl.MLP_NPARAM
+    assert open(relu, "rb").read(8) == b"FSLMLP02"
+    q, m, act = fsl.mlp_file_read_act(relu)
+    assert act == fsl.FSL_MLP_RELU and np.array_equal(p, q) and np.array_equal(n, m)
+    with pytest.raises(fsl.FslError):
+        fsl.mlp_file_read(relu)
+    tanh = str(tmp_path / "tanh.bin")
+    fsl.mlp_file_write_act(tanh, p, n, "tanh")          # written in the first format
+    assert open(tanh, "rb").read(8) == b"FSLMLP01"
+    assert fsl.mlp_file_read_act(tanh)[2] == fsl.FSL_MLP_TANH
+    q, m = fsl.mlp_file_read(tanh)
+    assert np.array_equal(p, q)
+    raw = bytearray(open(relu, "rb").read())
+    raw[20:24] = (7).to_bytes(4, "little")                # unknown activation
+    bad = str(tmp_path / "bad.bin")
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(fsl.FslError):
+        fsl.mlp_file_read_act(bad)
+    with pytest.raises(fsl.FslError):
+        fsl.mlp_file_write_act(bad, p, n, 5)
+
+
 def tanh_state(L, H, D, dw_cells, a=0.0):
     case = fsl.case_reference(L, H, D)
     dy = case.deltaY
