@@ -337,11 +337,12 @@ def main(argv=None):
     if "mlp_mu" in leg["prof"]:
         ms, cnt = leg["prof"]["mlp_mu"]
         tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
-        _, valu, mfma, prov = counter_figures("mlp_mu", a.precision, wl)
+        mtraffic, valu, mfma, prov = counter_figures("mlp_mu", a.precision, wl)
         mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": F16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tf / F16_PEAK_TFLOPS, 5),
                         "flop_per_cell": MLP_FLOP_PER_CELL, "mfma_flop_per_cell": MLP_MFMA_FLOP_PER_CELL,
                         "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": int(cnt),
+                        "traffic": mtraffic, "alg_bytes_per_launch": 40 * L * H * D,
                         "mfma_busy_pct": mfma, "valu_busy_pct": valu, "counters": prov}
     window = leg["window"]
     if a.sgs == "learned":
