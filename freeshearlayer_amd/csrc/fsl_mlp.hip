@@ -278,40 +278,6 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
 }
 
 // ---- features ----
-// Division of a 32-bit cell index n < 2^31 by a runtime divisor d: with
-// s = ceil(log2 d) and m = ceil(2^(31+s) / d) < 2^32, n / d =
-// (n m) >> (31 + s) exactly (the error of m / 2^(31+s) against 1/d stays
-// below 1/n); one 32 x 32 -> 64 multiply and a shift instead of an integer
-// division sequence.
-struct Div32 {
-    unsigned m;
-    int sh;
-};
-
-static Div32 div32(unsigned d)
-{
-    int s = 0;
-    while ((1ull << s) < d) s++;
-    const unsigned long long p = 1ull << (31 + s);
-    return Div32{(unsigned)((p + d - 1) / d), 31 + s};
-}
-
-__device__ __forceinline__ unsigned div_by(const Div32& q, unsigned n)
-{
-    return (unsigned)(((unsigned long long)n * q.m) >> q.sh);
-}
-
-struct CellDecode {
-    Div32 byD, byH;
-    Div32 byHb[2];   // row counts H/8 and H/8 + 1 of the y bands of k_mlp_mu_tr
-};
-
-static CellDecode cell_decode(const Geo& g)
-{
-    const unsigned hb = (unsigned)g.H / 8;
-    return CellDecode{div32((unsigned)g.D), div32((unsigned)g.H), {div32(hb ? hb : 1), div32(hb + 1)}};
-}
-
 struct FeatArgs {
     CState5 U;
     Geo g;
@@ -319,43 +285,54 @@ struct FeatArgs {
     double dU;        // velocity difference of the two streams
     const float* dw;  // vorticity thickness delta_w(x) per x index (L + 2)
     double dy;
-    CellDecode dec;   // interior index -> (i, j, k)
 };
 
-// Features of interior cell (i, j, k) from the 7-point stencil of u =
-// (rho u) / rho: invariants of the central-difference strain S and rotation
-// W, the grid-scale roughness Delta |lap u| / |S| (second differences), |S|
-// Delta / dU and y / delta_w(x).  Returns |S| = sqrt(2 S:S) (the model's
-// strain magnitude, src/turbulence.c:299-301).
-// Central-difference velocity gradient gr[r][e] = du_r/dx_e and the
-// second-difference Laplacian of u = (rho u) / rho at interior cell (i, j, k)
-// (FP32, hardware reciprocals), as in cell_features (which keeps its own copy
-// of the stencil: sharing this function cost the network kernel registers).
-__device__ __forceinline__ void velocity_gradient(const CState5& U, const Geo& g, const Phys& ph, int i, int j, int k,
-                                                  float gr[3][3], float lap[3])
+// Velocity u = (rho u) / rho of one stencil point in FP32: v_rcp_f32 of the
+// density and one product per component, each rounded (this function is not
+// under FP contraction), so every kernel that forms the features -- from a
+// plane ring in LDS or from global memory -- sees the same velocity values.
+__device__ __forceinline__ void point_velocity(double r, double m0, double m1, double m2, float u[3])
 {
-#pragma clang fp contract(fast)
-    const long long c = cidx(g, i, j, k);
-    const long long st[3] = {g.si, g.sj, 1};
-    const float rc = __builtin_amdgcn_rcpf((float)U.f[0][c]);
-    float uc[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        uc[r] = (float)U.f[1 + r][c] * rc;
-        lap[r] = 0.f;
-    }
+    const float rr = __builtin_amdgcn_rcpf((float)r);
+    u[0] = (float)m0 * rr;
+    u[1] = (float)m1 * rr;
+    u[2] = (float)m2 * rr;
+}
+
+// Velocity at a cell and its six neighbours: up[e] / um[e] the neighbour at
+// +1 / -1 along axis e (x, y, z).
+struct Stencil {
+    float uc[3], up[3][3], um[3][3];
+};
+
+// the stencil of cell c from global memory (strides st of x, y, z)
+__device__ __forceinline__ void gather_stencil(const CState5& U, long long c, const long long st[3], Stencil& s)
+{
+    point_velocity(U.f[0][c], U.f[1][c], U.f[2][c], U.f[3][c], s.uc);
 #pragma unroll
     for (int e = 0; e < 3; e++) {
         const long long p = c + st[e], m = c - st[e];
-        const float rp = __builtin_amdgcn_rcpf((float)U.f[0][p]);
-        const float rm = __builtin_amdgcn_rcpf((float)U.f[0][m]);
+        point_velocity(U.f[0][p], U.f[1][p], U.f[2][p], U.f[3][p], s.up[e]);
+        point_velocity(U.f[0][m], U.f[1][m], U.f[2][m], U.f[3][m], s.um[e]);
+    }
+}
+
+// Central-difference velocity gradient gr[r][e] = du_r/dx_e and the
+// second-difference Laplacian of u from a stencil (FP32, FMA contraction).
+__device__ __forceinline__ void stencil_gradient(const Stencil& s, const Phys& ph, float gr[3][3], float lap[3])
+{
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int r = 0; r < 3; r++) lap[r] = 0.f;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
         const float h = (float)ph.r2del[e];
         const float h2 = (float)(ph.rdel[e] * ph.rdel[e]);
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            const float up = (float)U.f[1 + r][p] * rp, um = (float)U.f[1 + r][m] * rm;
+            const float up = s.up[e][r], um = s.um[e][r];
             gr[r][e] = (up - um) * h;
-            lap[r] += (up - 2.f * uc[r] + um) * h2;
+            lap[r] += (up - 2.f * s.uc[r] + um) * h2;
         }
     }
 }
@@ -375,60 +352,19 @@ __device__ __forceinline__ float strain_sq(const float gr[3][3])
     return s2;
 }
 
-// State of the 7-point stencil of one cell for cell_features: ld(f, e, s)
-// is field f at the cell (s = 0) or its neighbour s = +-1 along axis e.
-struct PtrCell {   // plain pointers and 64-bit indices (any mesh)
-    CState5 U;
-    long long c, st[3];
-    __device__ __forceinline__ double ld(int f, int e, int s) const
-    {
-        return U.f[f][s == 0 ? c : (s > 0 ? c + st[e] : c - st[e])];
-    }
-};
-
-constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
-
-__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
-}
-
-// One buffer resource per field over the planes a workgroup's cells and
-// their neighbours occupy (scalar registers) and a 32-bit in-window byte
-// offset per lane: no per-load 64-bit address arithmetic.
-struct BufCell {
-    __amdgpu_buffer_rsrc_t r[4];
-    unsigned oc, stb[3];
-    __device__ __forceinline__ double ld(int f, int e, int s) const
-    {
-        return bload(r[f], s == 0 ? oc : (s > 0 ? oc + stb[e] : oc - stb[e]));
-    }
-};
-
-template <class Cell>
-__device__ __forceinline__ float cell_features(const Cell& q, const FeatArgs& a, int i, int j, float f[NIN])
+// Features of interior cell (i, j, k) from the 7-point stencil of u =
+// (rho u) / rho: invariants of the central-difference strain S and rotation
+// W, the grid-scale roughness Delta |lap u| / |S| (second differences), |S|
+// Delta / dU and y / delta_w(x).  Returns |S| = sqrt(2 S:S) (the model's
+// strain magnitude, src/turbulence.c:299-301).
+__device__ __forceinline__ float cell_features(const Stencil& st, const FeatArgs& a, int i, int j, float f[NIN])
 {
     // FP32 with FMA contraction and the hardware square root (1 ulp): the
-    // features reach the network rounded to BF16
+    // features reach the network rounded to FP16 (BF16 in training)
 #pragma clang fp contract(fast)
     const Geo& g = a.g;
-    const float rc = __builtin_amdgcn_rcpf((float)q.ld(0, 0, 0));
-    float uc[3], gr[3][3], lap[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int r = 0; r < 3; r++) uc[r] = (float)q.ld(1 + r, 0, 0) * rc;
-#pragma unroll
-    for (int e = 0; e < 3; e++) {
-        const float rp = __builtin_amdgcn_rcpf((float)q.ld(0, e, 1));
-        const float rm = __builtin_amdgcn_rcpf((float)q.ld(0, e, -1));
-        const float h = (float)a.ph.r2del[e];
-        const float h2 = (float)(a.ph.rdel[e] * a.ph.rdel[e]);
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const float up = (float)q.ld(1 + r, e, 1) * rp, um = (float)q.ld(1 + r, e, -1) * rm;
-            gr[r][e] = (up - um) * h;
-            lap[r] += (up - 2.f * uc[r] + um) * h2;
-        }
-    }
+    float gr[3][3], lap[3];
+    stencil_gradient(st, a.ph, gr, lap);
     // Invariants from the six entries of the symmetric S and the vorticity
     // om (W_ij = -eps_ijk om_k / 2, so W^2 = (om om^T - |om|^2 I) / 4):
     //   |W|^2 = |om|^2 / 2,  tr(S W^2) = (om.S.om - |om|^2 tr S) / 4,
@@ -472,14 +408,6 @@ __device__ __forceinline__ float cell_features(const Cell& q, const FeatArgs& a,
 }
 
 // interior cell t (k fastest) -> (i, j, k), 1-based
-__device__ __forceinline__ void decode_cell(const Geo& g, const CellDecode& c, unsigned t, int& i, int& j, int& k)
-{
-    const unsigned a = div_by(c.byD, t), b = div_by(c.byH, a);
-    k = (int)(t - a * (unsigned)g.D) + 1;
-    j = (int)(a - b * (unsigned)g.H) + 1;
-    i = (int)b + 1;
-}
-
 __device__ __forceinline__ void decode_interior(const Geo& g, long long t, int& i, int& j, int& k)
 {
     k = (int)(t % g.D) + 1;
@@ -497,28 +425,66 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
     int i, j, k;
     decode_interior(a.g, cell, i, j, k);
     float f[NIN];
-    const PtrCell q{a.U, cidx(a.g, i, j, k), {a.g.si, a.g.sj, 1}};
-    cell_features(q, a, i, j, f);
+    const long long st[3] = {a.g.si, a.g.sj, 1};
+    Stencil sv;
+    gather_stencil(a.U, cidx(a.g, i, j, k), st, sv);
+    cell_features(sv, a, i, j, f);
 #pragma unroll
     for (int q = 0; q < NIN; q++) X[s * NIN + q] = f[q];
     if (t) t[s] = cd[cidx(a.g, i, j, k)] / 0.15f;
 }
 
-// Learned mu_SGS, register-resident form.  The network runs transposed:
-// Z^T = W^T X^T with units as MFMA rows (M) and cells as columns (N).  The
-// output tile D[unit][cell] then sits in the lanes exactly as the next
-// layer's B operand B[unit][cell] (lane = cell + 16 (unit / 4), element =
-// unit % 4), so activations go from the accumulators through the activation
-// (act_r) and FP16 rounding straight into the next product.  The weight tiles are the
-// A operands (same lane layout as the packed B tiles) and stay in registers
-// for the block's whole grid-stride loop; only the 8 features of each cell
-// pass through LDS (one 8-byte read per lane and 16-cell tile).
-constexpr int MU_CELLS = 256;   // cells per block iteration (4 waves x 64)
+// Learned mu_SGS, register-resident network on a plane march.  A workgroup
+// (4 waves, one thread per cell) owns a 16 x 16 tile of (j, k) columns and
+// marches along x over a segment of planes.  The velocity of every point of
+// the tile and its one-cell ring (18 x 18) passes through a four-plane LDS
+// ring, so each point's four fields are read from global memory once per
+// plane (the ring's 68 points also by the neighbouring tiles, mostly from
+// L2) and its velocity is formed once; a cell's 7-point stencil comes from
+// the ring.  Each plane's loads are issued one iteration ahead.  One
+// workgroup barrier per plane: plane i + 2 goes into the ring slot of plane
+// i - 2, last read in iteration i - 1, before the barrier of iteration i.
+//
+// The network runs transposed: Z^T = W^T X^T with units as MFMA rows (M) and
+// cells as columns (N).  The output tile D[unit][cell] then sits in the lanes
+// exactly as the next layer's B operand B[unit][cell] (lane = cell + 16
+// (unit / 4), element = unit % 4), so activations go from the accumulators
+// through the activation (act4) and FP16 rounding straight into the next
+// product.  The weight tiles are the A operands (same lane layout as the
+// packed B tiles) and stay in registers for the whole march; only the 8
+// features of each cell pass through LDS (one 8-byte read per lane and
+// 16-cell tile).
+constexpr int MT = 16;                   // tile edge (j and k)
+constexpr int MW = MT + 2;               // tile and ring
+constexpr int MFP = MW * MW;             // 324 ring points
+constexpr int MHALO = MFP - MT * MT;     // 68 of them outside the tile
+constexpr int MRING = 4;                 // planes in the LDS ring
+static_assert(MT * MT == 256 && MHALO <= 256 && (MRING & (MRING - 1)) == 0, "mu kernel tiling");
+
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
+
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned boff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+}
+
+struct MuArgs {
+    FeatArgs f;
+    MlpW w;
+    double* mu;
+    float* cd_out;
+    float scale;
+    int segL, ntj, ntk, nblocks;
+};
 
 template <int TP, int ACT>   // 16-cell tiles in flight per layer; hidden activation
-__global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double* mu, float* cd_out, float scale)
+__global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
 {
+    __shared__ float s_u[MRING][3][MFP];   // velocity of the tile and ring, plane p in slot p % MRING
+    __shared__ double s_rho[MRING][MT * MT];   // density of the tile's cells (mu_SGS = rho Cd Delta^2 |S|)
     __shared__ unsigned short sX[4][64 * NIN];
+    const FeatArgs& a = ma.f;
+    const MlpW& w = ma.w;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q4 = lane >> 4;
     h4 A1[4];
@@ -546,51 +512,119 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
         ns[q] = w.norm[NIN + q];
     }
     unsigned short* X = sX[wv];
+
+    const Geo& g = a.g;
+    const int L = g.L, H = g.H, D = g.D;
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs
-    // (workgroup b on XCD b % 8), so XCD x takes the y band of rows
-    // [x H / 8, (x + 1) H / 8) and walks its cells plane by plane (k fastest,
-    // then j in the band, then i): the three planes of the stencil's band
-    // stay in that XCD's L2.  The grid is a multiple of 8.
-    const int band = blockIdx.x & 7;
-    const int jlo = band * a.g.H / 8, Hb = (band + 1) * a.g.H / 8 - jlo;
-    const Div32 byHb = a.dec.byHb[Hb == a.g.H / 8 ? 0 : 1];
-    const long long N = (long long)a.g.L * Hb * a.g.D;
-    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
-    const long long per = gridDim.x >> 3;
-    // band-local cell index -> (i, j, k), 1-based
-    auto decode = [&](unsigned t, int& ci, int& cj, int& ck) {
-        const unsigned q = div_by(a.dec.byD, t), r = div_by(byHb, q);
-        ck = (int)(t - q * (unsigned)a.g.D) + 1;
-        cj = jlo + (int)(q - r * (unsigned)Hb) + 1;
-        ci = (int)r + 1;
-    };
-    for (long long blk = blockIdx.x >> 3; blk < nblk; blk += per) {   // uniform over the workgroup
-        const long long cell = blk * MU_CELLS + threadIdx.x;
-        const bool valid = cell < N;
-        // the block's window: from the plane before its first cell's
-        int ib, jb, kb;
-        decode((unsigned)(blk * MU_CELLS), ib, jb, kb);
-        const long long base = (long long)(ib - 1) * a.g.si;
-        const long long wbytes = (a.g.G - base) * 8;
-        const int nrec = (int)(wbytes < 0x7fffffffLL ? wbytes : 0x7fffffffLL);
-        BufCell q;
+    // (workgroup b on XCD b % 8), so XCD x takes a contiguous range of tiles,
+    // whose shared ring points stay in its L2.  The grid is a multiple of 8.
+    int bid = blockIdx.x;
+    {
+        const int per = gridDim.x >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    if (bid >= ma.nblocks) return;   // the whole workgroup, before any barrier
+    const int tk = bid % ma.ntk;
+    const int rem = bid / ma.ntk;
+    const int tj = rem % ma.ntj;
+    const int seg = rem / ma.ntj;
+    const int j0 = 1 + tj * MT, k0 = 1 + tk * MT;
+    const int i0 = 1 + seg * ma.segL, i1 = min(i0 + ma.segL, L + 1);
+    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
+    const int j = j0 + tr, k = k0 + tc;
+    const bool valid = j <= H && k <= D;
+    const int po = (tr + 1) * MW + tc + 1;   // own point in the ring
+    // ring point outside the tile of threads t < MHALO: rows 0 and MW - 1,
+    // then columns 0 and MW - 1 of rows 1 .. MT
+    int jj, kk;
+    if (t < MW) {
+        jj = 0;
+        kk = t;
+    } else if (t < 2 * MW) {
+        jj = MW - 1;
+        kk = t - MW;
+    } else if (t < 2 * MW + MT) {
+        jj = t - 2 * MW + 1;
+        kk = 0;
+    } else {
+        jj = t - (2 * MW + MT) + 1;
+        kk = MW - 1;
+    }
+    const bool hal = t < MHALO;
+    const int ph = hal ? jj * MW + kk : po;
+    // in-plane byte offsets (rows and columns beyond the ghost layer clamped:
+    // only invalid lanes read those points)
+    const unsigned sjb = (unsigned)g.sj * 8u;
+    const unsigned oo = (unsigned)min(j, H + 1) * sjb + (unsigned)min(k, D + 1) * 8u;
+    const unsigned oh = (unsigned)min(j0 - 1 + jj, H + 1) * sjb + (unsigned)min(k0 - 1 + kk, D + 1) * 8u;
+    const int pb = (int)(g.si * 8);
+    auto ld4 = [&](int ip, unsigned off, double q[4]) {
+        const long long plane = (long long)min(ip, L + 1) * g.si;
 #pragma unroll
-        for (int fq = 0; fq < 4; fq++)
-            q.r[fq] = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.U.f[fq] + base), (short)0, nrec,
-                                                        kRsrcWord3);
-        q.stb[0] = (unsigned)a.g.si * 8u;
-        q.stb[1] = (unsigned)a.g.sj * 8u;
-        q.stb[2] = 8u;
+        for (int f = 0; f < 4; f++)
+            q[f] = bload(__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.U.f[f] + plane), (short)0, pb,
+                                                           kRsrcWord3),
+                         off);
+    };
+    auto put = [&](int ip, const double qo[4], const double qh[4]) {
+        const int sl = ip & (MRING - 1);
+        float u[3];
+        point_velocity(qo[0], qo[1], qo[2], qo[3], u);
+#pragma unroll
+        for (int e = 0; e < 3; e++) s_u[sl][e][po] = u[e];
+        s_rho[sl][t] = qo[0];
+        if (hal) {
+            point_velocity(qh[0], qh[1], qh[2], qh[3], u);
+#pragma unroll
+            for (int e = 0; e < 3; e++) s_u[sl][e][ph] = u[e];
+        }
+    };
+    double qo[4], qh[4] = {1., 0., 0., 0.};
+    {
+        double a0[4], a1[4], a2[4], h0[4] = {1., 0., 0., 0.}, h1[4] = {1., 0., 0., 0.}, h2[4] = {1., 0., 0., 0.};
+        ld4(i0 - 1, oo, a0);
+        ld4(i0, oo, a1);
+        ld4(i0 + 1, oo, a2);
+        if (hal) {
+            ld4(i0 - 1, oh, h0);
+            ld4(i0, oh, h1);
+            ld4(i0 + 1, oh, h2);
+        }
+        put(i0 - 1, a0, h0);
+        put(i0, a1, h1);
+        put(i0 + 1, a2, h2);
+    }
+    ld4(i0 + 2, oo, qo);
+    if (hal) ld4(i0 + 2, oh, qh);
+    for (int i = i0; i < i1; i++) {
+        lds_barrier();
+        Stencil st;
+        {
+            const int sm = (i - 1) & (MRING - 1), sc = i & (MRING - 1), sp = (i + 1) & (MRING - 1);
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                st.uc[r] = s_u[sc][r][po];
+                st.up[0][r] = s_u[sp][r][po];
+                st.um[0][r] = s_u[sm][r][po];
+                st.up[1][r] = s_u[sc][r][po + MW];
+                st.um[1][r] = s_u[sc][r][po - MW];
+                st.up[2][r] = s_u[sc][r][po + 1];
+                st.um[2][r] = s_u[sc][r][po - 1];
+            }
+        }
         float f[NIN], smag = 0.f;
-        int i = 1, j = 1, k = 1;
         if (valid) {
-            decode((unsigned)cell, i, j, k);
-            q.oc = ((unsigned)(i - ib + 1) * (unsigned)a.g.si + (unsigned)j * (unsigned)a.g.sj + (unsigned)k) * 8u;
-            smag = cell_features(q, a, i, j, f);
+            smag = cell_features(st, a, i, j, f);
         } else {
 #pragma unroll
             for (int q = 0; q < NIN; q++) f[q] = nm[q];
         }
+        const double rho = s_rho[i & (MRING - 1)][t];
+        // plane i + 2 (loaded during the previous iteration) into the ring,
+        // plane i + 3 in flight
+        put(i + 2, qo, qh);
+        ld4(i + 3, oo, qo);
+        if (hal) ld4(i + 3, oh, qh);
         float fn[NIN];
 #pragma unroll
         for (int q = 0; q < NIN; q++) fn[q] = clamp_h((f[q] - nm[q]) * ns[q]);
@@ -609,20 +643,20 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
             }
             h4 h1[TP][4], h2[TP][4];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int tt = 0; tt < 4; tt++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    const f4 z = mfma_h(A1[t], xb[u], f4{kb1[t][0], kb1[t][1], kb1[t][2], kb1[t][3]});
-                    h1[u][t] = act4<ACT>(z);
+                    const f4 z = mfma_h(A1[tt], xb[u], f4{kb1[tt][0], kb1[tt][1], kb1[tt][2], kb1[tt][3]});
+                    h1[u][tt] = act4<ACT>(z);
                 }
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int tt = 0; tt < 4; tt++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    f4 z = {kb2[t][0], kb2[t][1], kb2[t][2], kb2[t][3]};
-                    z = mfma_h32(A2[0][t], cat8(h1[u][0], h1[u][1]), z);
-                    z = mfma_h32(A2[1][t], cat8(h1[u][2], h1[u][3]), z);
-                    h2[u][t] = act4<ACT>(z);
+                    f4 z = {kb2[tt][0], kb2[tt][1], kb2[tt][2], kb2[tt][3]};
+                    z = mfma_h32(A2[0][tt], cat8(h1[u][0], h1[u][1]), z);
+                    z = mfma_h32(A2[1][tt], cat8(h1[u][2], h1[u][3]), z);
+                    h2[u][tt] = act4<ACT>(z);
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
@@ -635,27 +669,16 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_tr(FeatArgs a, MlpW w, double
             }
         }
         if (valid) {
-            const float Cd = fminf(fmaxf(0.15f * scale * ymine, 0.f), 0.15f);
-            const long long c = cidx(a.g, i, j, k);
-            mu[c] = q.ld(0, 0, 0) * (double)Cd * a.ph.DD * (double)smag;
-            if (cd_out) cd_out[c] = Cd;
+            const float Cd = fminf(fmaxf(0.15f * ma.scale * ymine, 0.f), 0.15f);
+            const long long c = cidx(g, i, j, k);
+            ma.mu[c] = rho * (double)Cd * a.ph.DD * (double)smag;
+            if (ma.cd_out) ma.cd_out[c] = Cd;
         }
         wave_lds_sync();   // X is rewritten by the next iteration
     }
 }
 
-int mu_variant()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("FSL_MLP_TILES");
-        v = e ? atoi(e) : 2;
-        if (v != 1 && v != 2 && v != 4) v = 2;
-    }
-    return v;
-}
-
-// forward pass on caller-given raw features (normalised as in k_mlp_mu_tr)
+// forward pass on caller-given raw features (normalised as in k_mlp_mu_plane)
 template <int ACT>
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, MlpW w, float* y)
 {
@@ -922,7 +945,6 @@ struct SumArgs {
     Phys ph;
     const double* mu_a;
     const double* mu_b;
-    CellDecode dec;
 };
 
 // SGS dissipation densities e = mu |S|^2 (|S| = sqrt(2 S:S) of the
@@ -941,8 +963,11 @@ __global__ void __launch_bounds__(256) k_sgs_dissipation_rows(SumArgs a, double*
     for (long long t = threadIdx.x; t < n; t += blockDim.x) {
         const int k = (int)(t % a.g.D) + 1;
         const int i = i0 + 1 + (int)(t / a.g.D);
+        const long long st[3] = {a.g.si, a.g.sj, 1};
+        Stencil sv;
+        gather_stencil(a.U, cidx(a.g, i, j, k), st, sv);
         float gr[3][3], lap[3];
-        velocity_gradient(a.U, a.g, a.ph, i, j, k, gr, lap);
+        stencil_gradient(sv, a.ph, gr, lap);
         const float sm = __builtin_amdgcn_sqrtf(2.f * strain_sq(gr));
         const long long c = cidx(a.g, i, j, k);
         const double ea = a.mu_a[c] * (double)sm * (double)sm, eb = a.mu_b[c] * (double)sm * (double)sm;
@@ -1059,36 +1084,37 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
                    double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out,
                    float scale, int act)
 {
-    FeatArgs a;
-    a.U = U;
-    a.g = g;
-    a.ph = ph;
-    a.dU = dU;
-    a.dw = dw;
-    a.dy = dy;
-    a.dec = cell_decode(g);
-    const long long N = (long long)g.L * g.H * g.D;
-    const long long nblk = (N + MU_CELLS - 1) / MU_CELLS;
-    static int ncu = 0;
-    if (!ncu) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-            ncu = 256;
+    MuArgs ma;
+    ma.f.U = U;
+    ma.f.g = g;
+    ma.f.ph = ph;
+    ma.f.dU = dU;
+    ma.f.dw = dw;
+    ma.f.dy = dy;
+    ma.w = mlpw(pack, master, norm);
+    ma.mu = mu;
+    ma.cd_out = cd_out;
+    ma.scale = scale;
+    ma.ntj = (g.H + MT - 1) / MT;
+    ma.ntk = (g.D + MT - 1) / MT;
+    const int cols = ma.ntj * ma.ntk;
+    int segL = 0;
+    // three workgroups per CU at this kernel's occupancy
+    const int nseg = fused_segments(g.L, cols, 3, &segL);
+    ma.segL = segL;
+    ma.nblocks = nseg * cols;
+    const dim3 grid((unsigned)((ma.nblocks + 7) / 8 * 8));   // a multiple of the 8 XCDs
+    static int tp = -1;
+    if (tp < 0) {
+        const char* e = getenv("FSL_MLP_TILES");
+        tp = e && atoi(e) == 2 ? 2 : 1;
     }
-    const int tp = mu_variant();
-    // one resident wave of blocks (3 per CU at this kernel's occupancy) loops over the
-    // cells; a multiple of the 8 XCDs (the kernel's order assumes it)
-    const dim3 grid((unsigned)std::max<long long>(8, std::min<long long>(nblk, 3LL * ncu) / 8 * 8));
-    const MlpW w = mlpw(pack, master, norm);
     if (act == kRelu) {
-        if (tp == 1) hipLaunchKernelGGL((k_mlp_mu_tr<1, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-        else if (tp == 4) hipLaunchKernelGGL((k_mlp_mu_tr<4, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-        else hipLaunchKernelGGL((k_mlp_mu_tr<2, kRelu>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        if (tp == 2) hipLaunchKernelGGL((k_mlp_mu_plane<2, kRelu>), grid, dim3(256), 0, s, ma);
+        else hipLaunchKernelGGL((k_mlp_mu_plane<1, kRelu>), grid, dim3(256), 0, s, ma);
     } else {
-        if (tp == 1) hipLaunchKernelGGL((k_mlp_mu_tr<1, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-        else if (tp == 4) hipLaunchKernelGGL((k_mlp_mu_tr<4, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
-        else hipLaunchKernelGGL((k_mlp_mu_tr<2, kTanh>), grid, dim3(256), 0, s, a, w, mu, cd_out, scale);
+        if (tp == 2) hipLaunchKernelGGL((k_mlp_mu_plane<2, kTanh>), grid, dim3(256), 0, s, ma);
+        else hipLaunchKernelGGL((k_mlp_mu_plane<1, kTanh>), grid, dim3(256), 0, s, ma);
     }
 }
 
@@ -1103,7 +1129,6 @@ void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph,
     a.dU = dU;
     a.dw = dw;
     a.dy = dy;
-    a.dec = cell_decode(g);
     hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, a, cd, stride,
                        offset, X, t, nout);
 }
@@ -1165,7 +1190,6 @@ void launch_sgs_dissipation(hipStream_t s, CState5 U, const Geo& g, const Phys& 
     a.ph = ph;
     a.mu_a = mu_a;
     a.mu_b = mu_b;
-    a.dec = cell_decode(g);
     hipLaunchKernelGGL(k_sgs_dissipation_rows, dim3(g.H, dissipation_splits(g)), dim3(256), 0, s, a, rows);
 }
 
