@@ -387,7 +387,9 @@ def main(argv=None):
         "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
                    "path": a.path, "mach": a.mach,
                    "closure": ({"model": os.path.relpath(a.model, ROOT),
-                                "activation": {fsl.FSL_MLP_TANH: "tanh", fsl.FSL_MLP_RELU: "relu"}[act]}
+                                "activation": {fsl.FSL_MLP_TANH: "tanh", fsl.FSL_MLP_RELU: "relu"}[act],
+                                "level": "set per y row by the judge (adapt 2)",
+                                "judge_interval": 50, "fallback_tol": 0.10}
                                if a.sgs == "learned" else None)},
         "roofline": roofline,
         "mlp_roofline": mlp_roofline,

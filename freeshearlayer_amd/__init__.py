@@ -191,6 +191,7 @@ def load():
         "fsl_learned_judge": (i_, [vp, _dp, _dp, _dp]),
         "fsl_learned_judge_errors": (i_, [vp, _dp, _dp, _dp, _dp, _dp]),
         "fsl_learned_set_tolerances": (i_, [vp, d_, d_]),
+        "fsl_learned_level_factors": (C.c_longlong, [vp, C.c_longlong, _fp]),
         "fsl_learned_history_errors": (i_, [vp, i_, _dp, _dp, _dp, _dp]),
         "fsl_mfma_probe": (i_, [vp, _fp, _fp, _fp]),
         "fsl_output_fields": (i_, [vp, i_, i_, _dp]),
@@ -529,8 +530,17 @@ class Context:
     def learned_configure(self, interval=50, tol=0.10):
         self._chk(self.lib.fsl_learned_configure(self.h, interval, tol), "learned_configure")
 
-    def learned_set_scale(self, scale=1.0, adapt=True):
+    def learned_set_scale(self, scale=1.0, adapt=2):
+        """adapt: 0 / False keeps the scale, 1 / True one global level set by
+        each judge, 2 a level per y row, 3 per x band and y row."""
         self._chk(self.lib.fsl_learned_set_scale(self.h, scale, int(adapt)), "learned_set_scale")
+
+    def learned_level_factors(self):
+        """Level factors f(i, j) of the network's scale, shape (LEN, HIG) (all 1 below adapt 2)."""
+        n = self.lib.fsl_learned_level_factors(self.h, 0, None)
+        out = np.zeros(n, np.float32)
+        self.lib.fsl_learned_level_factors(self.h, n, _fptr(out))
+        return out.reshape(self.case.LEN, self.case.HIG)
 
     def learned_history(self, errors=False):
         """Judge history: (errors, scales), or with errors=True (errors,

@@ -44,8 +44,14 @@ struct LearnedState {
     int interval = 50;         // judge every interval steps (0: never)
     double tol = 0.10;         // relative dissipation error that triggers the fallback
     int in_fallback = 0;
-    int adapt = 1;             // judge rescales the network's Cd to the dynamic model's dissipation
+    int adapt = 2;             // judge rescales the network's Cd to the dynamic model's dissipation:
+                               // 1 one global level, 2 a level per y row, 3 per x band and y row
     float scale = 1.f;         // current output scale of the network's Cd
+    float* fac = nullptr;      // adapt >= 2: factors of the scale per (plane i, row j), (L + 2)(H + 2) (device)
+    std::vector<float> hfac;   //   and their host copy
+    int nband = 1;             // x bands of the level factors (1 for adapt 2)
+    std::vector<float> band_f; // factor per (band, row j = 1 .. H)
+    std::vector<double> jrows; // last judge's per-row dissipation sums: dynamic (H), network (H)
     double last_error = -1.;
     double profile_tol = 1e30;       // bounds on the judge's profile and local errors (judge_error)
     double local_tol = 1e30;

@@ -88,9 +88,11 @@ int mlp_npack();   // packed BF16 operand entries (8 bytes each)
 void launch_mlp_pack(hipStream_t s, const float* master, void* pack, int act);
 void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* pack, const float* master,
                         const float* norm, float* y, int act);
+// fac: factors of scale per (plane i, row j), index i (H + 2) + j, or null
+// (all 1); fac_planes 0: every plane holds the same factors (plane 0's used)
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                    double dy, const void* pack, const float* master, const float* norm, double* mu,
-                   float* cd_out, float scale, int act);
+                   float* cd_out, float scale, const float* fac, int fac_planes, int act);
 void launch_mlp_features(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                          double dy, const float* cd, long long stride, long long offset, float* X, float* t,
                          long long nout);

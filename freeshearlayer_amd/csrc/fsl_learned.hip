@@ -12,7 +12,10 @@
 // fallback (dynamic model every step) until a later judge passes; otherwise
 // the network's mu_SGS is used.  Between judges the network alone runs.
 // The judge also refreshes delta_w(x), the thickness in the y / delta_w
-// feature, from the Favre-averaged profile of each x plane.
+// feature, from the Favre-averaged profile of each x plane, and sets the
+// level of the network's Cd for the following steps from the dynamic
+// model's dissipation: one global factor (adapt 1) or one per y row
+// (adapt 2).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -81,6 +84,65 @@ int ensure_judge(fsl_ctx* c)
     return FSL_OK;
 }
 
+// Level factors of the network's scale (adapt 2 and 3): band_f holds one
+// factor per (x band, row), the bands those of the judge's dissipation sums
+// (band s: planes s L / NB + 1 .. (s + 1) L / NB, integer division); the
+// kernel reads them per (plane, row), linear in x between band centres and
+// constant beyond the first and last centre (one band: the row's factor on
+// every plane).  Ghost planes and rows take their neighbour's factor.
+int upload_levels(fsl_ctx* c)
+{
+    LearnedState& L = c->lrn;
+    const int Lx = c->g.L, H = c->g.H, NB = L.nband;
+    const size_t n = (size_t)(Lx + 2) * (H + 2);
+    if (!L.fac) {
+        HIPCHK(c, hipMalloc(&L.fac, n * sizeof(float)));
+        c->bytes += n * sizeof(float);
+    }
+    L.hfac.assign(n, 1.f);
+    std::vector<double> cs(NB);
+    for (int b = 0; b < NB; b++)
+        cs[b] = 0.5 * ((double)((long long)b * Lx / NB) + 1. + (double)((long long)(b + 1) * Lx / NB));
+    for (int i = 1; i <= Lx; i++) {
+        int b0 = 0;
+        double w = 0.;
+        if (NB > 1) {
+            if (i <= cs[0]) {
+                b0 = 0;
+            } else if (i >= cs[NB - 1]) {
+                b0 = NB - 1;
+            } else {
+                while (b0 + 1 < NB - 1 && i > cs[b0 + 1]) b0++;
+                w = (i - cs[b0]) / (cs[b0 + 1] - cs[b0]);
+            }
+        }
+        for (int j = 1; j <= H; j++) {
+            const double f0 = L.band_f[(size_t)b0 * H + j - 1];
+            const double f = w > 0. ? (1. - w) * f0 + w * L.band_f[(size_t)(b0 + 1) * H + j - 1] : f0;
+            L.hfac[(size_t)i * (H + 2) + j] = (float)f;
+        }
+        L.hfac[(size_t)i * (H + 2)] = L.hfac[(size_t)i * (H + 2) + 1];
+        L.hfac[(size_t)i * (H + 2) + H + 1] = L.hfac[(size_t)i * (H + 2) + H];
+    }
+    for (int j = 0; j < H + 2; j++) {
+        L.hfac[j] = L.hfac[(size_t)(H + 2) + j];
+        L.hfac[(size_t)(Lx + 1) * (H + 2) + j] = L.hfac[(size_t)Lx * (H + 2) + j];
+    }
+    HIPCHK(c, hipMemcpyAsync(L.fac, L.hfac.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSL_OK;
+}
+
+// all level factors 1, with the band count of the adapt mode
+int reset_levels(fsl_ctx* c)
+{
+    LearnedState& L = c->lrn;
+    if (L.adapt < 2) return FSL_OK;
+    L.nband = L.adapt == 3 ? dissipation_splits(c->g) : 1;
+    L.band_f.assign((size_t)L.nband * c->g.H, 1.f);
+    return upload_levels(c);
+}
+
 int ensure_cd(fsl_ctx* c)
 {
     LearnedState& L = c->lrn;
@@ -121,7 +183,7 @@ int learned_mu(fsl_ctx* c, double* mu)
     }
     prof_mark(c, K_MLP_MU, 0);
     launch_mlp_mu(c->stream, cst5(c->U, c->g.G), c->g, c->ph, c->dU, L.dw, c->cs.deltaY, L.pack, L.master,
-                  L.norm, mu, nullptr, L.scale, L.act);
+                  L.norm, mu, nullptr, L.scale, L.adapt >= 2 ? L.fac : nullptr, L.nband > 1, L.act);
     prof_mark(c, K_MLP_MU, 1);
     return check_launch(c, "learned mu_SGS");
 }
@@ -171,6 +233,7 @@ int judge_error(fsl_ctx* c, const double* mu_a, const double* mu_b, JudgeErr* ou
     const double sa = tot[0], sb = tot[1];
     out->da = sa;
     out->db = sb;
+    L.jrows.assign(rows.begin(), rows.begin() + 2 * (size_t)H);
     if (sa > 0) out->err = fabs(sb - sa) / sa;
     else out->err = sb > 0 ? INFINITY : 0.;
     if (sa > 0 && sb > 0) {
@@ -229,6 +292,7 @@ void learned_free(fsl_ctx* c)
     (void)hipFree(L.dw);
     (void)hipFree(L.mu_alt);
     (void)hipFree(L.part);
+    (void)hipFree(L.fac);
     if (L.hpart) (void)hipHostFree(L.hpart);
     L = LearnedState();
 }
@@ -259,6 +323,23 @@ int learned_sgs_step(fsl_ctx* c)
         // (the network supplies the spatial distribution): used from the next step
         const float next = (L.adapt && dl > 0 && dd > 0) ? std::min(10.f, std::max(0.1f, (float)(L.scale * dd / dl)))
                                                          : L.scale;
+        // adapt 2 / 3: each (band, row) factor moves by its own ratio of the
+        // dynamic to the network's dissipation over the global one, so its
+        // level follows its own ratio; eps (1 % of the mean dissipation of a
+        // band's row) keeps rows of negligible dissipation, outside the layer,
+        // near their factor
+        if (L.adapt >= 2 && dl > 0 && dd > 0 && L.fac) {
+            const int H = c->g.H, NB = L.nband, ns = dissipation_splits(c->g);
+            const double g = dd / dl, eps = 0.01 * dd / ((double)NB * H);
+            for (int b = 0; b < NB; b++)
+                for (int j = 0; j < H; j++) {
+                    const double pa = NB == 1 ? L.jrows[j] : L.hpart[(size_t)b * H + j];
+                    const double pb = NB == 1 ? L.jrows[H + j] : L.hpart[((size_t)ns + b) * H + j];
+                    float& f = L.band_f[(size_t)b * H + j];
+                    f = (float)std::min(10., std::max(0.1, f * ((pa + eps) / (pb + eps) / g)));
+                }
+            if ((rc = upload_levels(c))) return rc;
+        }
         if (err > L.tol || perr > L.profile_tol || je.lerr > L.local_tol) {
             L.fallbacks++;
             L.in_fallback = 1;
@@ -301,6 +382,7 @@ int fsl_mlp_set_params(fsl_ctx* c, const float* params, const float* norm)
     L.ready = 1;
     L.adam_t = 0;
     L.scale = 1.f;   // the judge recalibrates the new network from scratch
+    if ((rc = reset_levels(c))) return rc;
     if (L.grad) {   // a new parameter set restarts the optimizer
         const size_t nb = mlp_nparam() * sizeof(float);
         HIPCHK(c, hipMemsetAsync(L.adam_m, 0, nb, c->stream));
@@ -324,7 +406,7 @@ int fsl_mlp_set_activation(fsl_ctx* c, int activation)
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     L.scale = 1.f;   // a different network: the judge recalibrates it
-    return FSL_OK;
+    return reset_levels(c);
 }
 
 int fsl_mlp_get_activation(const fsl_ctx* c) { return c->lrn.act; }
@@ -646,9 +728,22 @@ int fsl_learned_set_tolerances(fsl_ctx* c, double profile_tol, double local_tol)
 int fsl_learned_set_scale(fsl_ctx* c, double scale, int adapt)
 {
     if (!(scale >= 0.1 && scale <= 10.)) return set_err(c, FSL_ERR_ARG, "scale %g outside [0.1, 10]", scale);
+    if (adapt < 0 || adapt > 3) return set_err(c, FSL_ERR_ARG, "adapt %d (0 .. 3)", adapt);
     c->lrn.scale = (float)scale;
-    c->lrn.adapt = adapt ? 1 : 0;
-    return FSL_OK;
+    c->lrn.adapt = adapt;
+    return reset_levels(c);
+}
+
+long long fsl_learned_level_factors(const fsl_ctx* c, long long nmax, float* out)
+{
+    const LearnedState& L = c->lrn;
+    const int Lx = c->g.L, H = c->g.H;
+    const bool on = L.adapt >= 2 && L.fac;
+    long long q = 0;
+    for (int i = 1; i <= Lx; i++)
+        for (int j = 1; j <= H; j++, q++)
+            if (q < nmax) out[q] = on ? L.hfac[(size_t)i * (H + 2) + j] : 1.f;
+    return (long long)Lx * H;
 }
 
 int fsl_learned_history(const fsl_ctx* c, int nmax, double* err, double* scale)

@@ -474,6 +474,8 @@ struct MuArgs {
     double* mu;
     float* cd_out;
     float scale;
+    const float* fac;    // factor of scale per (plane i, row j), index i fsi + j (or null)
+    int fsi;             // plane stride of fac: H + 2, or 0 where the factors do not depend on x
     int segL, ntj, ntk, nblocks;
 };
 
@@ -596,7 +598,10 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
     }
     ld4(i0 + 2, oo, qo);
     if (hal) ld4(i0 + 2, oh, qh);
+    const float scale0 = ma.fac && valid ? ma.scale * ma.fac[j] : ma.scale;
     for (int i = i0; i < i1; i++) {
+        // level of the network's Cd in this cell (used after the network)
+        const float scale = ma.fac && ma.fsi && valid ? ma.scale * ma.fac[(long long)i * ma.fsi + j] : scale0;
         lds_barrier();
         Stencil st;
         {
@@ -669,7 +674,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
             }
         }
         if (valid) {
-            const float Cd = fminf(fmaxf(0.15f * ma.scale * ymine, 0.f), 0.15f);
+            const float Cd = fminf(fmaxf(0.15f * scale * ymine, 0.f), 0.15f);
             const long long c = cidx(g, i, j, k);
             ma.mu[c] = rho * (double)Cd * a.ph.DD * (double)smag;
             if (ma.cd_out) ma.cd_out[c] = Cd;
@@ -1082,7 +1087,7 @@ void launch_mlp_forward(hipStream_t s, const float* X, long long n, const void* 
 
 void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, double dU, const float* dw,
                    double dy, const void* pack, const float* master, const float* norm, double* mu, float* cd_out,
-                   float scale, int act)
+                   float scale, const float* fac, int fac_planes, int act)
 {
     MuArgs ma;
     ma.f.U = U;
@@ -1095,6 +1100,8 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     ma.mu = mu;
     ma.cd_out = cd_out;
     ma.scale = scale;
+    ma.fac = fac;
+    ma.fsi = fac_planes ? g.H + 2 : 0;
     ma.ntj = (g.H + MT - 1) / MT;
     ma.ntk = (g.D + MT - 1) / MT;
     const int cols = ma.ntj * ma.ntk;

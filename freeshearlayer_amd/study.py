@@ -63,6 +63,7 @@ def run(path, mesh, args, model):
     if sgs == "learned":
         ctx.mlp_load(model)
         ctx.learned_configure(args.interval, args.tol)
+        ctx.learned_set_scale(1.0, args.adapt)
     rec = {"path": path, "sgs": sgs, "precision": prec, "step": [], "delta_theta": [], "delta_theta_out": [],
            "delta_w": [], "tke": [], "mass": [], "energy": [], "dt": []}
 
@@ -108,6 +109,7 @@ def run(path, mesh, args, model):
         e, sc = ctx.learned_history()
         rec["judge_error"] = e.tolist()
         rec["judge_scale"] = sc.tolist()
+        rec["adapt"] = args.adapt
     S = ctx.download_state() if not args.ref_steps else None
     ctx.close()
     return rec, S
@@ -127,6 +129,9 @@ def main(argv=None):
     ap.add_argument("--model", default=DEFAULT_MODEL)
     ap.add_argument("--interval", type=int, default=50)
     ap.add_argument("--tol", type=float, default=0.10)
+    ap.add_argument("--adapt", type=int, default=2, choices=[0, 1, 2, 3],
+                    help="level of the network's Cd set by each judge: 0 kept, 1 global, 2 per y row, "
+                         "3 per x band and y row")
     ap.add_argument("--spectra", action="store_true")
     ap.add_argument("--ref-steps", type=int, default=0,
                     help="steps of the reference leg when shorter than the candidate's (compared over those)")

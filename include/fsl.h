@@ -414,14 +414,26 @@ int fsl_mlp_gradient(fsl_ctx *ctx, long long batch, float *grad, float *loss);
    exceeds tol, or the profile or local error its bound
    (fsl_learned_set_tolerances; default 1e30: reported, not enforced), the
    step runs on the dynamic model, and so does every step until a later
-   judge passes.  With adapt on, each judge also sets the network's output
+   judge passes.  With adapt 1, each judge also sets the network's output
    scale (Cd = clip(0.15 scale y, 0, 0.15)) to scale D_d / D_l, clamped to
    [0.1, 10], for the following steps: the network gives the spatial
-   distribution of Cd, the dynamic procedure its global level.  Defaults:
-   interval 50, tol 0.10, scale 1, adapt on; interval 0 never judges. */
+   distribution of Cd, the dynamic procedure its global level.  With adapt 2
+   the level is set per y row, with adapt 3 per x band and y row (the bands
+   of the judge's sums: NB = min(16, LEN), band b the planes b LEN / NB + 1
+   .. (b + 1) LEN / NB): cell (i, j)'s scale is scale f(i, j), and each judge
+   multiplies the factor of band b and row j by (P_d(b, j) + eps) /
+   (P_l(b, j) + eps) / (D_d / D_l), eps = 1 % of the mean dissipation of a
+   band's row, clamped to [0.1, 10] (rows outside the layer, of negligible
+   dissipation, keep their factor); f(i, j) is linear in x between the band
+   centres (adapt 2: one band).  Defaults: interval 50, tol 0.10, scale 1,
+   adapt 2; interval 0 never judges; adapt 0 keeps the scale. */
 int fsl_learned_configure(fsl_ctx *ctx, int interval, double tol);
 int fsl_learned_set_tolerances(fsl_ctx *ctx, double profile_tol, double local_tol);
+/* sets the scale and the adapt mode (0 .. 3); adapt 2 and 3 reset the level factors to 1 */
 int fsl_learned_set_scale(fsl_ctx *ctx, double scale, int adapt);
+/* level factors f(i, j) of the interior, out[(i - 1) HIG + j - 1] for
+   q < nmax (all 1 below adapt 2); returns LEN HIG */
+long long fsl_learned_level_factors(const fsl_ctx *ctx, long long nmax, float *out);
 typedef struct fsl_learned_status {
     int interval;
     double tol;

@@ -331,3 +331,63 @@ def test_judge_rescales_network_level():
         assert ctx.learned_status()["scale"] == 2.0
         with pytest.raises(fsl.FslError):
             ctx.learned_set_scale(20.0)
+
+
+def band_level_factors(P_d, P_l, L, nb, ratio_floor=0.01):
+    """The level factors after one judge from factors 1 (adapt 2: nb = 1;
+    adapt 3: nb bands), from per-(band, row) dissipation sums P_d, P_l
+    (nb x H): f(b, j) = clip((P_d + eps) / (P_l + eps) / (D_d / D_l), 0.1,
+    10), eps = 1 % of the mean (band, row) sum, then per plane linear in x
+    between the band centres."""
+    g = P_d.sum() / P_l.sum()
+    eps = ratio_floor * P_d.sum() / P_d.size
+    fb = np.clip((P_d + eps) / (P_l + eps) / g, 0.1, 10.0)
+    centres = np.array([0.5 * (b * L // nb + 1 + (b + 1) * L // nb) for b in range(nb)])
+    planes = np.arange(1, L + 1)
+    if nb == 1:
+        return np.repeat(fb, L, 0), g
+    f = np.stack([np.interp(planes, centres, fb[:, j]) for j in range(fb.shape[1])], 1)
+    return f, g
+
+
+@pytest.mark.parametrize("adapt", [2, 3])
+def test_judge_sets_level_factors(adapt):
+    """adapt 2 / 3: a judge sets the global scale to D_dynamic / D_learned
+    and the factor of each y row (adapt 2) or x band and row (adapt 3) to
+    its own ratio over the global one (band_level_factors); P(b, j), the
+    x-z sums of mu |S|^2, here from numpy's FP64 |S| (the device forms |S|
+    in FP32: rtol 1e-4)."""
+    L, H, D = 24, 20, 16
+    ctx, case, U, u_in = shear_ctx(L, H, D, sgs=fsl.FSL_SGS_LEARNED, fused=1)
+    p, _ = fsl.mlp_init_params(29)
+    with ctx:
+        ctx.ghost_fill()
+        U0 = ctx.download_state()
+        X = ctx.mlp_features()
+        norm = np.concatenate([X.mean(0), 1 / (X.std(0) + 1e-12)]).astype(np.float32)
+        ctx.mlp_set_params(p, norm)
+        ctx.learned_configure(interval=1, tol=1e30)
+        ctx.learned_set_scale(1.0, adapt=adapt)
+        assert np.all(ctx.learned_level_factors() == 1.0)
+        ctx.mlp_update_thickness()
+        ctx.dynamic_smagorinsky()
+        mu_d = ctx.download_mu()
+        ctx.learned_smagorinsky()
+        mu_l = ctx.download_mu()
+        ctx.step()                                 # judged on the step-start state
+        f = ctx.learned_level_factors()
+        st = ctx.learned_status()
+    gr, _ = mlp_ref.velocity_gradient(U0, case)
+    S = 0.5 * (gr + np.swapaxes(gr, -1, -2))
+    s2 = 2 * (S * S).sum((-1, -2))                 # |S|^2, (L, H, D)
+    ed = (mu_d[1:-1, 1:-1, 1:-1] * s2).sum(2)      # x-y map of the z sums
+    el = (mu_l[1:-1, 1:-1, 1:-1] * s2).sum(2)
+    nb = 1 if adapt == 2 else min(16, L)
+    bands = [slice(b * L // nb, (b + 1) * L // nb) for b in range(nb)]
+    Pd = np.stack([ed[s].sum(0) for s in bands])
+    Pl = np.stack([el[s].sum(0) for s in bands])
+    expect, g = band_level_factors(Pd, Pl, L, nb)
+    print("factors, row 10 along x:", np.round(f[:, 10], 3))
+    np.testing.assert_allclose(f, expect, rtol=1e-4)
+    assert abs(st["scale"] - min(10.0, max(0.1, g))) < 1e-4 * g
+    assert np.ptp(f) > 0.05                        # the levels do differ
