@@ -172,8 +172,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
             if (yface) m_y = bload(mr, omy);
             if (zface) m_z = bload(mr, omz);
         }
+        // step state of the cell this iteration updates (i - 1): interior lanes only
         double bq[5];
-        if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, oc, bq);
+        if (STAGE > 1 && interior && i > i0) load5<PK>(a.base, (long long)(i - 1) * si, pb, fb, oc, bq);
         lds_barrier();   // the previous plane's LDS readers are done
         const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {

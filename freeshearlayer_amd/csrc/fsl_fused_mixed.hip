@@ -166,9 +166,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         load5<PK>(a.in, (long long)min(i + 1, L + 1) * si, pb, fb, colb, w2);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
         const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colmu) : 0.;
-        // step state of the cell this iteration updates (i - 1)
+        // step state of the cell this iteration updates (i - 1): interior lanes only
         double bq[5];
-        if (STAGE > 1) load5<PK>(a.base, (long long)max(i - 1, 0) * si, pb, fb, colb, bq);
+        if (STAGE > 1 && interior && i > i0) load5<PK>(a.base, (long long)(i - 1) * si, pb, fb, colb, bq);
         // y neighbours of this plane's cell (written to s_w in the previous
         // iteration); z neighbours from the lanes beside it, and at a row's
         // ends the outer neighbour loaded in the previous iteration
