@@ -38,18 +38,22 @@ struct MixedArgs {
 __device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
 
 // v of the lane after / before this one in its row of 16 lanes (DPP
-// row_shl:1 / row_shr:1 on both halves; the row's last / first lane gets 0).
+// row_shl:1 / row_shr:1 on both halves); the row's last / first lane, whose
+// source lies outside the row, keeps `edge` (bound_ctrl off: the DPP move
+// leaves such a lane's destination, preloaded with `edge`, unchanged).
 // Every lane of the wave must execute these.
 template <int CTRL>
-__device__ __forceinline__ double row_dpp(double v)
+__device__ __forceinline__ double row_dpp(double v, double edge)
 {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+    const unsigned long long e = __builtin_bit_cast(unsigned long long, edge);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(e >> 32), (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf,
+                                               false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
-__device__ __forceinline__ double row_from_next(double v) { return row_dpp<0x101>(v); }
-__device__ __forceinline__ double row_from_prev(double v) { return row_dpp<0x111>(v); }
+__device__ __forceinline__ double row_from_next(double v, double edge) { return row_dpp<0x101>(v, edge); }
+__device__ __forceinline__ double row_from_prev(double v, double edge) { return row_dpp<0x111>(v, edge); }
 
 template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
@@ -179,9 +183,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             for (int f = 0; f < 5; f++) {
                 nyp[f] = s_w[sl][f][cw + BK];
                 nym[f] = s_w[sl][f][cw - BK];
-                const double zn = row_from_next(w1[f]), zb = row_from_prev(w1[f]);   // all lanes
-                nzp[f] = tx == BK - 1 ? pze[f] : zn;
-                nzm[f] = tx == 0 ? pze[f] : zb;
+                // all lanes; the row's end lanes keep their outer neighbour
+                nzp[f] = row_from_next(w1[f], pze[f]);
+                nzm[f] = row_from_prev(w1[f], pze[f]);
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
             if (yedge) load5<PK>(a.in, pn, pb, fb, oye, pye);
