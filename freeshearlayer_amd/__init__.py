@@ -21,6 +21,7 @@ FSL_OK = 0
 # error codes of include/fsl.h
 FSL_ERR_ARG, FSL_ERR_HIP, FSL_ERR_NOMEM, FSL_ERR_NONFINITE, FSL_ERR_IO, FSL_ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 FSL_SGS_CONSTANT, FSL_SGS_DYNAMIC, FSL_SGS_LEARNED = 0, 1, 2
+FSL_SGS_AVG_NONE, FSL_SGS_AVG_Z, FSL_SGS_AVG_XZ = 0, 1, 2
 FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
 FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1
 FSL_MLP_TANH, FSL_MLP_RELU = 0, 1
@@ -136,6 +137,8 @@ def load():
         "fsl_rk_update": (i_, [vp, i_, i_]),
         "fsl_check_co_num": (i_, [vp, _dp]),
         "fsl_dynamic_smagorinsky": (i_, [vp]),
+        "fsl_set_sgs_average": (i_, [vp, i_]),
+        "fsl_get_sgs_average": (i_, [vp]),
         "fsl_filter": (i_, [vp, _dp, _dp, _dp]),
         "fsl_stage": (i_, [vp, i_]),
         "fsl_step": (i_, [vp]),
@@ -373,6 +376,13 @@ class Context:
     def evolution(self, n, s): self._chk(self.lib.fsl_evolution(self.h, n, s), "evolution")
     def rk_update(self, scheme, s): self._chk(self.lib.fsl_rk_update(self.h, scheme, s), "rk_update")
     def dynamic_smagorinsky(self): self._chk(self.lib.fsl_dynamic_smagorinsky(self.h), "dynamic")
+
+    def set_sgs_average(self, mode):
+        """Averaging of the dynamic coefficient: FSL_SGS_AVG_NONE (pointwise, the
+        reference's), FSL_SGS_AVG_Z (z lines) or FSL_SGS_AVG_XZ (x-z planes)."""
+        self._chk(self.lib.fsl_set_sgs_average(self.h, int(mode)), "set_sgs_average")
+
+    def sgs_average(self): return self.lib.fsl_get_sgs_average(self.h)
     def stage(self, s): self._chk(self.lib.fsl_stage(self.h, s), "stage")
     def step(self): self._chk(self.lib.fsl_step(self.h), "step")
     def run(self, n): self._chk(self.lib.fsl_run(self.h, n), "run")

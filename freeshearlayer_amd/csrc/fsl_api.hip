@@ -134,6 +134,17 @@ Faces faces_of(fsl_ctx* c)
     return F;
 }
 
+const SgsAvg* sgs_avg_of(fsl_ctx* c)
+{
+    static thread_local SgsAvg a;
+    if (c->sgs_avg == FSL_SGS_AVG_NONE || !c->avg_buf) return nullptr;
+    a.mode = c->sgs_avg;
+    a.lm = c->avg_buf;
+    a.mm = c->avg_buf + c->g.G;
+    a.sums = c->avg_buf + 2 * c->g.G;
+    return &a;
+}
+
 int ensure_sgs(fsl_ctx* c)
 {
     // the FP64 model keeps its intermediates on chip; the FP32 model (mixed
@@ -347,6 +358,7 @@ void fsl_destroy(fsl_ctx* c)
     (void)hipFree(c->sums);
     (void)hipFree(c->red);
     (void)hipFree(c->probe_buf);
+    (void)hipFree(c->avg_buf);
     if (c->red_host) (void)hipHostFree(c->red_host);
     learned_free(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -562,11 +574,27 @@ int fsl_dynamic_smagorinsky(fsl_ctx* c)
     int rc = ensure_sgs(c);
     if (rc) return rc;
     if (c->cfg.precision == FSL_PREC_MIXED)
-        launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, nullptr);
+        launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, nullptr,
+                       sgs_avg_of(c));
     else
-        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, c->g, c->ph, prof_mark, c, nullptr);
+        launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, c->g, c->ph, prof_mark, c, nullptr, sgs_avg_of(c));
     return check_launch(c, "dynamic_smagorinsky");
 }
+
+int fsl_set_sgs_average(fsl_ctx* c, int mode)
+{
+    if (mode != FSL_SGS_AVG_NONE && mode != FSL_SGS_AVG_Z && mode != FSL_SGS_AVG_XZ)
+        return set_err(c, FSL_ERR_ARG, "averaging mode %d", mode);
+    if (mode != FSL_SGS_AVG_NONE && !c->avg_buf) {
+        const size_t n = sgs_avg_bytes(c->g);
+        HIPCHK(c, hipMalloc(&c->avg_buf, n));
+        c->bytes += n;
+    }
+    c->sgs_avg = mode;
+    return FSL_OK;
+}
+
+int fsl_get_sgs_average(const fsl_ctx* c) { return c->sgs_avg; }
 
 int fsl_fluxes(fsl_ctx* c)
 {

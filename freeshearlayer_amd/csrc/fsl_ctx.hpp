@@ -82,6 +82,8 @@ struct fsl_ctx {
     unsigned long long* red;
     unsigned long long* red_host;   // pinned copy of the CFL maximum (hipHostMalloc)
     double* probe_buf;  // 6 HIG probe values (allocated on first use)
+    int sgs_avg;        // FSL_SGS_AVG_*: averaging of the dynamic coefficient
+    double* avg_buf;    // its terms and sums (sgs_avg_bytes; allocated on first use)
     size_t bytes;
 
     int cur_is_p;       // stage input: U (0) or Up (1)
@@ -113,6 +115,8 @@ void prof_mark(void* arg, int id, int end);
 State5 st5(double* base, long long G);
 CState5 cst5(const double* base, long long G);
 int ensure_sgs(fsl_ctx* c);
+// the averaging of the dynamic model in force (null: pointwise)
+const SgsAvg* sgs_avg_of(fsl_ctx* c);
 int check_launch(fsl_ctx* c, const char* what);
 Bnd bnd_now(fsl_ctx* c);
 int ctx_ghost(fsl_ctx* c, const double* state, bool z = true);

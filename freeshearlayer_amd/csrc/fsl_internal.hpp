@@ -2,6 +2,7 @@
 // units and the context layer (fsl_api.hip).  Not part of the C-ABI.
 #pragma once
 
+#include "fsl.h"
 #include "fsl_device.hpp"
 
 namespace fsl {
@@ -66,17 +67,33 @@ int sums_nblocks(const Geo& g);
 void launch_copy(hipStream_t s, const double* a, double* b, size_t n);
 
 // --- dynamic Smagorinsky (fsl_sgs.hip) ---
+// Averaged coefficient (opt-in, not the reference's pointwise model): the
+// model kernels write the Germano terms L:M and M:M of every interior cell to
+// lm, mm and rho Delta^2 |S| to mu; the averaging kernels sum the terms over
+// z lines (mode FSL_SGS_AVG_Z, sums per (i, j)) or x-z planes
+// (FSL_SGS_AVG_XZ, per j) in a fixed order, form Cd = -1/2 sum(L:M) /
+// (sum(M:M) + small) clipped to [0, 0.15], and scale mu by it.  sums holds
+// 2 LEN HIG + 2 HIG doubles.  A null SgsAvg (or mode 0): pointwise Cd.
+struct SgsAvg {
+    int mode;
+    double* lm;
+    double* mm;
+    double* sums;
+};
+size_t sgs_avg_bytes(const Geo& g);
+void launch_sgs_average(hipStream_t s, const SgsAvg& av, double* mu, float* cd_out, const Geo& g);
 // FP64: the reference's operations in one x-marching kernel without
 // intermediate arrays (fsl_sgs_march.hip)
 void launch_sgs(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph,
-                void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
+                void (*prof)(void*, int, int), void* prof_arg, float* cd_out, const SgsAvg* avg = nullptr);
 // bytes of scratch the FP32 model needs (16 float cell arrays)
 size_t sgs_f32_scratch_bytes(const Geo& g);
-void launch_sgs_march(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, float* cd_out);
+void launch_sgs_march(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, float* cd_out,
+                      double* lm = nullptr, double* mm = nullptr);
 // the same model in FP32 (filters, Germano contraction) with two tiled
 // kernels and 16 float scratch arrays; mu_SGS stays an FP64 array
 void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
-                    void (*prof)(void*, int, int), void* prof_arg, float* cd_out);
+                    void (*prof)(void*, int, int), void* prof_arg, float* cd_out, const SgsAvg* avg = nullptr);
 
 // x segments of the fused stage kernels (fsl_fused.hip): count and length
 int fused_segments(int L, int cols, int per_cu, int* seg_len);

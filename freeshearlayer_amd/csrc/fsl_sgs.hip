@@ -27,6 +27,8 @@
 //            M = Delta^2 (4 B - A^), Cd and mu_SGS.
 // Every intermediate value is computed by the same operations on the same
 // inputs as the pass-by-pass form, so mu_SGS is bit-identical to it.
+#include <algorithm>
+
 #include "fsl_internal.hpp"
 
 namespace fsl {
@@ -373,6 +375,8 @@ struct SgsB {
     const R* mag;
     double* mu;
     float* cd;   // optional: the clipped coefficient (training targets)
+    double* lm;  // averaged model: L:M and M:M per cell (mu then holds rho Delta^2 |S|), or null
+    double* mm;
 };
 
 template <class R>
@@ -440,6 +444,12 @@ __global__ void __launch_bounds__(kBlock) k_sgs_b(SgsB<R> a, Geo g, Phys ph)
                              Lm[5] * M[5] + Lm[6] * M[6] + Lm[7] * M[7] + Lm[8] * M[8]);
         const R MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
                              M[5] * M[5] + M[6] * M[6] + M[7] * M[7] + M[8] * M[8]);
+        if (a.lm) {   // averaged model: the terms, and the factor of the averaged Cd
+            a.lm[c] = (double)LLMM;
+            a.mm[c] = (double)MMMM;
+            a.mu[c] = a.rho[c] * ph.DD * (double)a.mag[c];
+            continue;
+        }
         R Cd = -(R)0.5 * div_r(LLMM, MMMM + (R)kSmall);
         if (Cd > (R)0.15) Cd = (R)0.15;
         else if (Cd < (R)0.0) Cd = (R)0.0;
@@ -486,9 +496,74 @@ void launch_filter(hipStream_t s, const double* in, double* out, double* t1, dou
 
 namespace {
 
+// Averaged coefficient: z-line sums of L:M and M:M per (i, j) -- one
+// wavefront per line, a lane-strided sum and a fixed butterfly, so the sums
+// are reproducible -- into sums[(i - 1) H + j - 1] and sums[L H + ...].
+__global__ void __launch_bounds__(256) k_avg_lines(const double* lm, const double* mm, Geo g, double* sums)
+{
+    const long long line = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (line >= (long long)g.L * g.H) return;
+    const int i = (int)(line / g.H) + 1, j = (int)(line % g.H) + 1;
+    const long long c0 = cidx(g, i, j, 0);
+    double a = 0., b = 0.;
+    for (int k = 1 + lane; k <= g.D; k += 64) {
+        a += lm[c0 + k];
+        b += mm[c0 + k];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if (lane == 0) {
+        sums[line] = a;
+        sums[(long long)g.L * g.H + line] = b;
+    }
+}
+
+// x-z plane sums per j from the line sums, over i in order, into
+// sums[2 L H + j - 1] and sums[2 L H + H + j - 1]
+__global__ void k_avg_planes(Geo g, double* sums)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.H) return;
+    const long long LH = (long long)g.L * g.H;
+    double a = 0., b = 0.;
+    for (int i = 0; i < g.L; i++) {
+        a += sums[(long long)i * g.H + j];
+        b += sums[LH + (long long)i * g.H + j];
+    }
+    sums[2 * LH + j] = a;
+    sums[2 * LH + g.H + j] = b;
+}
+
+// Cd from the sums of the cell's line or plane, clipped to [0, 0.15]; mu
+// (rho Delta^2 |S| on entry) times Cd
+__global__ void __launch_bounds__(256) k_avg_apply(double* mu, float* cd, const double* sums, Geo g, int mode)
+{
+    const long long n = (long long)g.L * g.H * g.D;
+    const long long LH = (long long)g.L * g.H;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.D) + 1;
+        const long long line = t / g.D;
+        const int j = (int)(line % g.H) + 1, i = (int)(line / g.H) + 1;
+        const double a = mode == FSL_SGS_AVG_Z ? sums[line] : sums[2 * LH + j - 1];
+        const double b = mode == FSL_SGS_AVG_Z ? sums[LH + line] : sums[2 * LH + g.H + j - 1];
+        double Cd = -0.5 * (a / (b + kSmall));
+        if (Cd > 0.15) Cd = 0.15;
+        else if (Cd < 0.0) Cd = 0.0;
+        const long long c = cidx(g, i, j, k);
+        mu[c] *= Cd;
+        if (cd) cd[c] = (float)Cd;
+    }
+}
+
 template <class R>
 void sgs_pair(hipStream_t s, CState5 U, double* mu, R* const uf[3], R* const Lij[6], R* const Aij[6], R* mag,
-              const Geo& g, const Phys& ph, void (*prof)(void*, int, int), void* pa, float* cd_out)
+              const Geo& g, const Phys& ph, void (*prof)(void*, int, int), void* pa, float* cd_out,
+              const SgsAvg* avg)
 {
     const int ntiles = ((g.L + TI - 1) / TI) * ((g.H + TJ - 1) / TJ) * ((g.D + TK - 1) / TK);
     SgsA<R> A{};
@@ -512,25 +587,47 @@ void sgs_pair(hipStream_t s, CState5 U, double* mu, R* const uf[3], R* const Lij
     Bk.mag = mag;
     Bk.mu = mu;
     Bk.cd = cd_out;
+    const bool av = avg && avg->mode != FSL_SGS_AVG_NONE;
+    Bk.lm = av ? avg->lm : nullptr;
+    Bk.mm = av ? avg->mm : nullptr;
     if (prof) prof(pa, K_SGS_FINAL, 0);
     hipLaunchKernelGGL(k_sgs_b<R>, dim3(ntiles), dim3(kBlock), 0, s, Bk, g, ph);
+    if (av) launch_sgs_average(s, *avg, mu, cd_out, g);
     if (prof) prof(pa, K_SGS_FINAL, 1);
 }
 
 }  // namespace
 
-void launch_sgs(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, void (*prof)(void*, int, int),
-                void* pa, float* cd_out)
+size_t sgs_avg_bytes(const Geo& g)
 {
+    return (2 * (size_t)g.G + 2 * (size_t)g.L * g.H + 2 * (size_t)g.H) * sizeof(double);
+}
+
+void launch_sgs_average(hipStream_t s, const SgsAvg& av, double* mu, float* cd_out, const Geo& g)
+{
+    const long long LH = (long long)g.L * g.H;
+    hipLaunchKernelGGL(k_avg_lines, dim3((unsigned)((LH + 3) / 4)), dim3(256), 0, s, av.lm, av.mm, g, av.sums);
+    if (av.mode == FSL_SGS_AVG_XZ)
+        hipLaunchKernelGGL(k_avg_planes, dim3((unsigned)((g.H + 63) / 64)), dim3(64), 0, s, g, av.sums);
+    const long long n = LH * g.D;
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_avg_apply, dim3(nb), dim3(256), 0, s, mu, cd_out, av.sums, g, av.mode);
+}
+
+void launch_sgs(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, void (*prof)(void*, int, int),
+                void* pa, float* cd_out, const SgsAvg* avg)
+{
+    const bool av = avg && avg->mode != FSL_SGS_AVG_NONE;
     if (prof) prof(pa, K_SGS_FINAL, 0);
-    launch_sgs_march(s, U, mu, g, ph, cd_out);
+    launch_sgs_march(s, U, mu, g, ph, cd_out, av ? avg->lm : nullptr, av ? avg->mm : nullptr);
+    if (av) launch_sgs_average(s, *avg, mu, cd_out, g);
     if (prof) prof(pa, K_SGS_FINAL, 1);
 }
 
 size_t sgs_f32_scratch_bytes(const Geo& g) { return (size_t)(3 + 6 + 6 + 1) * (size_t)g.G * sizeof(float); }
 
 void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const Geo& g, const Phys& ph,
-                    void (*prof)(void*, int, int), void* pa, float* cd_out)
+                    void (*prof)(void*, int, int), void* pa, float* cd_out, const SgsAvg* avg)
 {
     // 16 float cell arrays: u^ (3), Leonard stress (6), A (6), |S|
     float* p = (float*)scratch;
@@ -538,7 +635,7 @@ void launch_sgs_f32(hipStream_t s, CState5 U, double* mu, void* scratch, const G
     for (int r = 0; r < 3; r++) { uf[r] = p; p += g.G; }
     for (int m = 0; m < 6; m++) { Lij[m] = p; p += g.G; }
     for (int m = 0; m < 6; m++) { Aij[m] = p; p += g.G; }
-    sgs_pair<float>(s, U, mu, uf, Lij, Aij, p, g, ph, prof, pa, cd_out);
+    sgs_pair<float>(s, U, mu, uf, Lij, Aij, p, g, ph, prof, pa, cd_out, avg);
 }
 
 }  // namespace fsl

@@ -52,6 +52,8 @@ struct MarchArgs {
     CState5 U;
     double* mu;
     float* cd;
+    double* lm;   // averaged model: L:M and M:M per cell (mu then holds rho Delta^2 |S|), or null
+    double* mm;
     Geo g;
     Phys ph;
     int segL, ntj, ntk, nblocks, xcd_remap;
@@ -414,12 +416,18 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
                                  Lm[5] * M[5] + Lm[6] * M[6] + Lm[7] * M[7] + Lm[8] * M[8]);
             const double MMMM = (M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + M[3] * M[3] + M[4] * M[4] +
                                  M[5] * M[5] + M[6] * M[6] + M[7] * M[7] + M[8] * M[8]);
-            double Cd = -0.5 * (LLMM / (MMMM + kSmall));
-            if (Cd > 0.15) Cd = 0.15;
-            else if (Cd < 0.0) Cd = 0.0;
             const long long c = cidx(g, PM, gjc, gkc);
-            a.mu[c] = rho2 * Cd * a.ph.DD * mag_prev;
-            if (a.cd) a.cd[c] = (float)Cd;
+            if (a.lm) {   // averaged model: the terms, and the factor of the averaged Cd
+                a.lm[c] = LLMM;
+                a.mm[c] = MMMM;
+                a.mu[c] = rho2 * a.ph.DD * mag_prev;
+            } else {
+                double Cd = -0.5 * (LLMM / (MMMM + kSmall));
+                if (Cd > 0.15) Cd = 0.15;
+                else if (Cd < 0.0) Cd = 0.0;
+                a.mu[c] = rho2 * Cd * a.ph.DD * mag_prev;
+                if (a.cd) a.cd[c] = (float)Cd;
+            }
         }
         // rotate the per-column pipeline
 #pragma unroll
@@ -434,12 +442,15 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
 
 }  // namespace
 
-void launch_sgs_march(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, float* cd_out)
+void launch_sgs_march(hipStream_t s, CState5 U, double* mu, const Geo& g, const Phys& ph, float* cd_out,
+                      double* lm, double* mm)
 {
     MarchArgs a{};
     a.U = U;
     a.mu = mu;
     a.cd = cd_out;
+    a.lm = lm;
+    a.mm = mm;
     a.g = g;
     a.ph = ph;
     a.ntj = (g.H + MJ - 1) / MJ;

@@ -142,6 +142,17 @@ typedef struct fsl_ctx fsl_ctx;
 #define FSL_SGS_DYNAMIC  1   /* src/turbulence.c:103-507, as shipped */
 #define FSL_SGS_LEARNED  2   /* per-cell MLP for Cs^2, dynamic model as judge */
 
+/* Averaging of the dynamic coefficient (fsl_set_sgs_average).  The
+   reference's model is pointwise (NONE, the default and the parity mode).
+   Z: Cd(i, j) = -1/2 sum_k L:M / (sum_k M:M + 1e-30) over each z line;
+   XZ: Cd(j) from the sums over each x-z plane (the "plane averaging" of the
+   Germano-Lilly procedure); both clipped to [0, 0.15], mu_SGS = rho Cd
+   Delta^2 |S|.  Applies to the dynamic model wherever it runs (the dynamic
+   path and the learned closure's judge and fallback). */
+#define FSL_SGS_AVG_NONE 0
+#define FSL_SGS_AVG_Z    1
+#define FSL_SGS_AVG_XZ   2
+
 #define FSL_PREC_FP64    0   /* parity path: FP64, no contraction, reference op order */
 #define FSL_PREC_MIXED   1   /* FP32 stencils and SGS, FP64 state / RK / p recovery / CFL */
 
@@ -222,6 +233,11 @@ int fsl_check_co_num(fsl_ctx *ctx, double *maxCo);
    the reference's operations; mixed-precision contexts: the same model with
    FP32 filters, contraction and scratch) */
 int fsl_dynamic_smagorinsky(fsl_ctx *ctx);
+/* Averaging of the dynamic coefficient, FSL_SGS_AVG_* (no reference
+   counterpart: the reference's model is pointwise; default NONE).  The
+   averaged modes keep 2 G + 2 LEN HIG + 2 HIG doubles of terms and sums. */
+int fsl_set_sgs_average(fsl_ctx *ctx, int mode);
+int fsl_get_sgs_average(const fsl_ctx *ctx);
 /* src/turbulence.c:17-100 with kernel h (the model uses {1/3,1/3,1/3}):
    host cell array in -> host cell array out (the returned array of the
    reference) */

@@ -680,7 +680,11 @@ void or_filter(const or_params *p, const double *in, double *out, const double h
     free(t2);
 }
 
-void or_dynamic_smagorinsky(const or_params *p, double *const U[5], double *mu_sgs)
+/* src/turbulence.c:103-507.  With lm (and mm, fac) given, also the Germano
+ * terms L:M, M:M and rho DD |S| of every interior cell (the pointwise model's
+ * intermediates, from which the averaged coefficients are formed). */
+static void dynamic_core(const or_params *p, double *const U[5], double *mu_sgs, double *lm, double *mm,
+                         double *fac)
 {
     /* src/turbulence.c:103-507, Germano-Lilly, pointwise Cd clipped to [0,0.15] */
     const int L = p->L, H = p->H, D = p->D;
@@ -768,10 +772,25 @@ void or_dynamic_smagorinsky(const or_params *p, double *const U[5], double *mu_s
                 Cd = -0.5 * (LLMM / (MMMM + SMALL));
                 if (Cd > 0.15) Cd = 0.15;
                 else if (Cd < 0.0) Cd = 0.0;
-                mu_sgs[c] = U[0][c] * Cd * p->DD * mag[c];
+                if (mu_sgs) mu_sgs[c] = U[0][c] * Cd * p->DD * mag[c];
+                if (lm) {
+                    lm[c] = LLMM;
+                    mm[c] = MMMM;
+                    fac[c] = U[0][c] * p->DD * mag[c];
+                }
             }
     free(u); free(v); free(w); free(uf); free(vf); free(wf); free(mag);
     for (a = 0; a < 9; a++) { free(prod[a]); free(fil[a]); free(A[a]); free(Af[a]); free(B[a]); }
+}
+
+void or_dynamic_smagorinsky(const or_params *p, double *const U[5], double *mu_sgs)
+{
+    dynamic_core(p, U, mu_sgs, NULL, NULL, NULL);
+}
+
+void or_dynamic_terms(const or_params *p, double *const U[5], double *lm, double *mm, double *fac)
+{
+    dynamic_core(p, U, NULL, lm, mm, fac);
 }
 
 void or_step(or_params *p, or_state *s)
@@ -948,6 +967,10 @@ void or_sim_evolution(or_sim *m, int n, int st) { or_evolution(&m->p, &m->s, n, 
 double or_sim_check_co_num(or_sim *m) { return or_check_co_num(&m->p, &m->s); }
 void or_sim_block_draw(or_sim *m) { or_block_draw(&m->p); }
 void or_sim_dynamic(or_sim *m) { or_dynamic_smagorinsky(&m->p, m->s.U, m->s.mu_sgs); }
+void or_sim_dynamic_terms(or_sim *m, double *lm, double *mm, double *fac)
+{
+    or_dynamic_terms(&m->p, m->s.U, lm, mm, fac);
+}
 void or_sim_filter(or_sim *m, const double *in, double *out)
 {
     const double h[3] = { 1. / 3., 1. / 3., 1. / 3. };

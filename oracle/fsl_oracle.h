@@ -111,6 +111,9 @@ void or_evolution(const or_params *p, or_state *s, int numStages, int stage);
 /* filter_ (src/turbulence.c:17-100): out must be zero-filled on entry */
 void or_filter(const or_params *p, const double *in, double *out, const double h[3]);
 void or_dynamic_smagorinsky(const or_params *p, double *const U[5], double *mu_sgs);
+/* the same model's Germano terms L:M, M:M and rho DD |S| per interior cell
+   (G arrays; test reference of the averaged coefficients) */
+void or_dynamic_terms(const or_params *p, double *const U[5], double *lm, double *mm, double *fac);
 
 /* One time step in the order of src/layer2.c:141-185 (probes/output left out). */
 void or_step(or_params *p, or_state *s);
@@ -147,6 +150,7 @@ void    or_sim_evolution(or_sim *s, int numStages, int stage);
 double  or_sim_check_co_num(or_sim *s);
 void    or_sim_block_draw(or_sim *s);
 void    or_sim_dynamic(or_sim *s);
+void    or_sim_dynamic_terms(or_sim *s, double *lm, double *mm, double *fac);
 void    or_sim_filter(or_sim *s, const double *in, double *out);
 void    or_sim_step(or_sim *s);
 void    or_sim_ic_reference(or_sim *s);

@@ -69,6 +69,7 @@ class Oracle:
             lib.or_sim_check_co_num.restype = C.c_double
             lib.or_sim_filter.argtypes = [C.c_void_p, _dp, _dp]
             lib.or_sim_output_fields.argtypes = [C.c_void_p, _dp]
+            lib.or_sim_dynamic_terms.argtypes = [C.c_void_p, _dp, _dp, _dp]
             lib.or_sim_probe_values.argtypes = [C.c_void_p, _dp]
             lib.or_sim_cur_is_p.argtypes = [C.c_void_p]
             lib.or_sim_cur_is_p.restype = C.c_int
@@ -136,6 +137,14 @@ class Oracle:
     def check_co_num(self): return self.lib().or_sim_check_co_num(self.h)
     def block_draw(self): self.lib().or_sim_block_draw(self.h)
     def dynamic(self): self.lib().or_sim_dynamic(self.h)
+
+    def dynamic_terms(self):
+        """The dynamic model's L:M, M:M and rho DD |S| per cell (interior set)."""
+        shp = cell_shape(self.L, self.H, self.D)
+        lm, mm, fac = (np.zeros(shp) for _ in range(3))
+        self.lib().or_sim_dynamic_terms(self.h, _ptr(lm), _ptr(mm), _ptr(fac))
+        return lm, mm, fac
+
     def step(self): self.lib().or_sim_step(self.h)
     def ic_reference(self): self.lib().or_sim_ic_reference(self.h)
     def cur_is_p(self): return self.lib().or_sim_cur_is_p(self.h)

@@ -251,3 +251,21 @@ def test_z_quirk_is_what_the_reference_does():
     orc.check_co_num(); orc.block_draw(); orc.ghost(); orc.recon()
     got = np.concatenate([orc.face(f).ravel() for f in range(30)])
     assert not np.array_equal(got, g["faces_recon"])
+
+
+def test_dynamic_terms_reproduce_the_pointwise_model():
+    """The oracle's Germano terms (the test reference of the averaged
+    coefficients) give its pointwise mu_SGS: clip(-L:M / 2 (M:M + 1e-30), 0,
+    0.15) rho DD |S| (to rounding: the pointwise model associates rho Cd DD |S|)."""
+    L, H, D = 16, 12, 8
+    o = Oracle(L, H, D, dynamic=1)
+    o.set_state(random_state(L, H, D, seed=41))
+    o.ghost()
+    o.dynamic()
+    mu = o.cell(10).copy()
+    lm, mm, fac = o.dynamic_terms()
+    Cd = np.clip(-0.5 * (lm / (mm + 1e-30)), 0.0, 0.15)
+    I = (slice(1, -1),) * 3
+    assert np.abs(mu[I] - Cd[I] * fac[I]).max() <= 1e-14 * np.abs(mu).max()
+    assert (Cd[I] > 0).any() and (Cd[I] < 0.15).any() and (Cd[I] == 0).any()
+    assert np.all(lm[0] == 0) and np.all(fac[:, 0] == 0)    # ghost layer untouched
