@@ -131,7 +131,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
-    double w0[5], w1[5], w2[5];
+    // stage input of the current and the next plane (that of the previous
+    // plane, which the update needs, is read back from the plane copy)
+    double w1[5], w2[5];
     double pye[5], pze[5];                   // edge lanes: outer y/z neighbours of the next plane
 #pragma unroll
     for (int f = 0; f < 5; f++) pze[f] = 0.;
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 
     for (int i = i0 - 1; i <= i1; i++) {
 #pragma unroll
-        for (int f = 0; f < 5; f++) { w0[f] = w1[f]; w1[f] = w2[f]; }
+        for (int f = 0; f < 5; f++) w1[f] = w2[f];
         // Loads in the order of first use (the wait counter retires in issue
         // order): next plane's cell and this cell's mu_SGS (end of the
         // reconstruction), the step state of the updated cell (after the
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // y neighbours of this plane's cell (written to s_w in the previous
         // iteration); z neighbours from the lanes beside it, and at a row's
         // ends the outer neighbour loaded in the previous iteration
-        double nyp[5], nym[5], nzp[5], nzm[5];
+        double nyp[5], nym[5], nzp[5], nzm[5], w0[5];
         {
             const int sl = i & 1;
 #pragma unroll
@@ -188,16 +190,17 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 nzm[f] = row_from_prev(w1[f], pze[f]);
             }
             const long long pn = (long long)min(i + 1, L + 1) * si;
+            // stage input of the cell the update below writes (plane i - 1)
+#pragma unroll
+            for (int f = 0; f < 5; f++) w0[f] = s_w[(i + 1) & 1][f][cw];
             if (yedge) load5<PK>(a.in, pn, pb, fb, oye, pye);
             if (zedge) load5<PK>(a.in, pn, pb, fb, oze, pze);
         }
         // No barrier here: the LDS written before the next barrier (s_vr of
         // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
-        // minus states) was last read by the previous plane's face loops, and
-        // the s_w slot of plane i + 1 (= slot of i - 1) at the start of the
-        // previous plane, all before the previous plane's last barrier;
-        // s_fy/s_fz, read after that barrier, are rewritten only after the
-        // next one.
+        // minus states) was last read by the previous plane's face loops,
+        // before the previous plane's last barrier; s_fy/s_fz, read after
+        // that barrier, are rewritten only after the next one.
         const int vc = i & 1, vb = vc ^ 1;
         mx::PrimF P;
         mx::cell_prims(w1, ri1, v1, P);
@@ -243,8 +246,6 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             }
             ri1 = ri;
             s_mu[vc][c] = mu_c;
-#pragma unroll
-            for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cw] = w2[f];
         }
         lds_barrier();
 
@@ -376,8 +377,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
             }
         }
-        // outer y neighbours of plane i + 1 (the slot's last reads were the
-        // previous plane's, before its barriers)
+        // plane i + 1 into the plane copy (the slot of plane i - 1: its y
+        // neighbours were read at the start of the previous plane, before its
+        // barriers, and this lane's own cell in the update above), with the
+        // outer y neighbours that the edge rows loaded
+#pragma unroll
+        for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cw] = w2[f];
         if (yedge) {
 #pragma unroll
             for (int f = 0; f < 5; f++) s_w[(i + 1) & 1][f][cwy] = pye[f];
