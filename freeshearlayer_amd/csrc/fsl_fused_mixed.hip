@@ -35,8 +35,6 @@ struct MixedArgs {
     int cfl_mode;
 };
 
-__device__ __forceinline__ int ring(int p) { return ((p % 3) + 3) % 3; }
-
 // v of the lane after / before this one in its row of 16 lanes (DPP
 // row_shl:1 / row_shr:1 on both halves); the row's last / first lane, whose
 // source lies outside the row, keeps `edge` (bound_ctrl off: the DPP move
@@ -59,7 +57,8 @@ template <int STAGE, int BJ, int BK, int MINW, bool PK>
 __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 {
     constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK, NW = (BJ + 2) * BK;
-    __shared__ float s_vr[3][3][NT];    // 1/rho velocities, planes i-1, i, i+1
+    __shared__ float s_vr[2][3][NT];    // 1/rho velocities of planes i, i + 1 (slot p & 1)
+    __shared__ float s_d[3][3][NT];     // their central differences along x, y, z at plane i
     __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
     __shared__ float s_ym[4][NT];       // - face states along y: density, momenta
     __shared__ double s_ymE[NT];        //   and energy
@@ -123,6 +122,10 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const int cw = (ty + 1) * BK + tx;
     const bool yedge = ty == 0 || ty == BJ - 1, zedge = tx == 0 || tx == BK - 1;
     const int cwy = ty == 0 ? cw - BK : cw + BK;
+    // in-tile neighbours of the central differences (clamped at the tile's
+    // edges, where the differences are not used)
+    const int oyp = ty < BJ - 1 ? c + BK : c, oym = ty > 0 ? c - BK : c;
+    const int ozp = tx < BK - 1 ? c + 1 : c, ozm = tx > 0 ? c - 1 : c;
     const int kie = (((tx == 0 ? k - 1 : k + 1) - 1) % D + D) % D + 1;
     const unsigned oye = (unsigned)(ty == 0 ? jm : jp) * sjb + (unsigned)kim * 8u;
     const unsigned oze = (unsigned)jc * sjb + (unsigned)kie * 8u;
@@ -139,6 +142,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     for (int f = 0; f < 5; f++) pze[f] = 0.;
     float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
     float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
+    float vm1[3] = {0.f, 0.f, 0.f};          // velocity of the previous plane's cell
+    float Dy_prev[3] = {0.f, 0.f, 0.f}, Dz_prev[3] = {0.f, 0.f, 0.f};   // y / z differences, plane i - 1
     mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
 #pragma unroll
@@ -153,13 +158,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
         for (int f = 0; f < 5; f++) dxp[f] = (float)(w2[f] - w1[f]);
         ri1 = mx::rcp((float)w2[0]);
-        const int r = ring(ib);
 #pragma unroll
         for (int e = 0; e < 3; e++) {
             v1[e] = (float)w2[1 + e] * ri1;
-            s_vr[r][e][c] = v1[e];
+            s_vr[ib & 1][e][c] = v1[e];
         }
     }
+    lds_barrier();   // the first plane's velocities, for its y / z differences
 
     for (int i = i0 - 1; i <= i1; i++) {
 #pragma unroll
@@ -196,9 +201,20 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             if (yedge) load5<PK>(a.in, pn, pb, fb, oye, pye);
             if (zedge) load5<PK>(a.in, pn, pb, fb, oze, pze);
         }
+        // y and z central differences of the velocity at this plane (the
+        // tangential gradients of the faces are sums of two of them)
+        float Dx[3], Dy[3], Dz[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            Dy[q] = s_vr[i & 1][q][oyp] - s_vr[i & 1][q][oym];
+            Dz[q] = s_vr[i & 1][q][ozp] - s_vr[i & 1][q][ozm];
+            s_d[1][q][c] = Dy[q];
+            s_d[2][q][c] = Dz[q];
+        }
         // No barrier here: the LDS written before the next barrier (s_vr of
-        // plane i + 1 = ring slot of i - 2, s_visc/s_mu slot of i - 2, the y/z
-        // minus states) was last read by the previous plane's face loops,
+        // plane i + 1, read for the previous plane's differences before its
+        // first barrier; s_d, s_visc/s_mu slot of i - 2 and the y/z minus
+        // states, read by the previous plane's face loops) was last read
         // before the previous plane's last barrier; s_fy/s_fz, read after
         // that barrier, are rewritten only after the next one.
         const int vc = i & 1, vb = vc ^ 1;
@@ -236,13 +252,17 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             for (int f = 0; f < 5; f++) dxp[f] = dp[f];
         }
         {
-            // 1/rho and velocity of plane i + 1: LDS ring, and this cell's next iteration
+            // 1/rho and velocity of plane i + 1 (LDS, and this cell's next
+            // iteration); the x difference of this plane
             const float ri = mx::rcp((float)w2[0]);
-            const int r = ring(i + 1);
 #pragma unroll
             for (int e = 0; e < 3; e++) {
-                v1[e] = (float)w2[1 + e] * ri;
-                s_vr[r][e][c] = v1[e];
+                const float vn = (float)w2[1 + e] * ri;
+                Dx[e] = vn - vm1[e];
+                s_d[0][e][c] = Dx[e];
+                vm1[e] = v1[e];
+                v1[e] = vn;
+                s_vr[(i + 1) & 1][e][c] = vn;
             }
             ri1 = ri;
             s_mu[vc][c] = mu_c;
@@ -266,17 +286,14 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             }
             float R, vel[3], Pf;
             mx::riemann<0>(l, r, R, vel, Pf);
-            const int rc = ring(i), rb = ring(i - 1);
             const float vf[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
             const float vbk[3] = {s_visc[vb][0][c], s_visc[vb][1][c], s_visc[vb][2][c]};
             float gr[3][3];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 gr[q][0] = (vf[q] - vbk[q]) * rd[0];
-                gr[q][1] = (s_vr[rc][q][c + BK] + s_vr[rb][q][c + BK] - s_vr[rc][q][c - BK] -
-                            s_vr[rb][q][c - BK]) * r4[1];
-                gr[q][2] = (s_vr[rc][q][c + 1] + s_vr[rb][q][c + 1] - s_vr[rc][q][c - 1] -
-                            s_vr[rb][q][c - 1]) * r4[2];
+                gr[q][1] = (Dy[q] + Dy_prev[q]) * r4[1];
+                gr[q][2] = (Dz[q] + Dz_prev[q]) * r4[2];
             }
             float mu_dyn = 0.f;
             if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c] + s_mu[vb][c]));
@@ -308,9 +325,13 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
         }
         xp_prev = xp;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            Dy_prev[q] = Dy[q];
+            Dz_prev[q] = Dz[q];
+        }
 
         if (inseg) {
-            const int rp = ring(i + 1), rc = ring(i), rm = ring(i - 1);
             const float vbk[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
             const float Tb = s_visc[vc][3][c], Rb = s_visc[vc][4][c];
             if (yface) {
@@ -338,10 +359,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     gr[q][1] = (vf[q] - vbk[q]) * rd[1];
-                    gr[q][0] = (s_vr[rp][q][c + BK] + s_vr[rp][q][c] - s_vr[rm][q][c + BK] - s_vr[rm][q][c]) *
-                               r4[0];
-                    gr[q][2] = (s_vr[rc][q][c + BK + 1] + s_vr[rc][q][c + 1] - s_vr[rc][q][c + BK - 1] -
-                                s_vr[rc][q][c - 1]) * r4[2];
+                    gr[q][0] = (s_d[0][q][c + BK] + Dx[q]) * r4[0];
+                    gr[q][2] = (s_d[2][q][c + BK] + Dz[q]) * r4[2];
                 }
                 float mu_dyn = 0.f;   // cell (i, j + 1) and its x neighbour (src/fluxes.c:360)
                 if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + BK] + s_mu[vb][c + BK]));
@@ -364,9 +383,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     gr[q][2] = (vf[q] - vbk[q]) * rd[2];
-                    gr[q][0] = (s_vr[rp][q][c + 1] + s_vr[rp][q][c] - s_vr[rm][q][c + 1] - s_vr[rm][q][c]) * r4[0];
-                    gr[q][1] = (s_vr[rc][q][c + 1 + BK] + s_vr[rc][q][c + BK] - s_vr[rc][q][c + 1 - BK] -
-                                s_vr[rc][q][c - BK]) * r4[1];
+                    gr[q][0] = (s_d[0][q][c + 1] + Dx[q]) * r4[0];
+                    gr[q][1] = (s_d[1][q][c + 1] + Dy[q]) * r4[1];
                 }
                 float mu_dyn = 0.f;   // cell (i, k + 1) and its x neighbour (src/fluxes.c:531)
                 if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + 1] + s_mu[vb][c + 1]));
