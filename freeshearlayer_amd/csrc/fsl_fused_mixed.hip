@@ -59,14 +59,14 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     constexpr int TJ = BJ - 2, TK = BK - 2, NT = BJ * BK, NW = (BJ + 2) * BK;
     __shared__ float s_vr[2][3][NT];    // 1/rho velocities of planes i, i + 1 (slot p & 1)
     __shared__ float s_d[3][3][NT];     // their central differences along x, y, z at plane i
-    __shared__ float s_visc[2][5][NT];  // velocity (3), T, rho of planes i-1, i
+    __shared__ float s_visc[5][NT];     // velocity (3), T, rho of plane i
     __shared__ float s_ym[4][NT];       // - face states along y: density, momenta
     __shared__ double s_ymE[NT];        //   and energy
     __shared__ float s_zm[4][NT];       // - face states along z
     __shared__ double s_zmE[NT];
     __shared__ float s_fy[5][NT];       // y flux through the + face
     __shared__ float s_fz[5][NT];       // z flux through the + face
-    __shared__ double s_mu[2][NT];      // mu_SGS of planes i-1, i (slot i & 1)
+    __shared__ float s_mu[NT];          // mu_SGS of the faces to the cell's x neighbour: (mu(i) + mu(i - 1)) / 2
     __shared__ double s_w[2][5][NW];    // state of planes i, i+1 (slot p & 1) with the rows of outer
                                         // neighbours above and below the tile: y neighbours
 
@@ -143,6 +143,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     float dxp[5];                            // x difference q(i) - q(i - 1) of the current plane
     float ri1, v1[3];                        // 1/rho and velocity of the current plane's cell
     float vm1[3] = {0.f, 0.f, 0.f};          // velocity of the previous plane's cell
+    float vm2[3] = {0.f, 0.f, 0.f};          //   and of the one before
+    float Tprev = 0.f, Rprev = 0.f;          // temperature and density of the previous plane's cell
+    double mu_prev = 0.;                     // its mu_SGS
     float Dy_prev[3] = {0.f, 0.f, 0.f}, Dz_prev[3] = {0.f, 0.f, 0.f};   // y / z differences, plane i - 1
     mx::FaceF xp_prev;
     float Fx_prev[5], ly[5], lz[5];
@@ -217,14 +220,15 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         // states, read by the previous plane's face loops) was last read
         // before the previous plane's last barrier; s_fy/s_fz, read after
         // that barrier, are rewritten only after the next one.
-        const int vc = i & 1, vb = vc ^ 1;
+        float mu_sum = 0.f;
         mx::PrimF P;
         mx::cell_prims(w1, ri1, v1, P);
-        s_visc[vc][0][c] = v1[0];
-        s_visc[vc][1][c] = v1[1];
-        s_visc[vc][2][c] = v1[2];
-        s_visc[vc][3][c] = mx::cell_temp(P);
-        s_visc[vc][4][c] = (float)w1[0];
+        const float Tcur = mx::cell_temp(P), Rcur = (float)w1[0];
+        s_visc[0][c] = v1[0];
+        s_visc[1][c] = v1[1];
+        s_visc[2][c] = v1[2];
+        s_visc[3][c] = Tcur;
+        s_visc[4][c] = Rcur;
         // the y/z reconstruction needs nothing loaded in this iteration; the
         // uses of w2 and mu_c follow it, so that it covers their latency
         const float qf[4] = {(float)w1[0], (float)w1[1], (float)w1[2], (float)w1[3]};
@@ -260,12 +264,14 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 const float vn = (float)w2[1 + e] * ri;
                 Dx[e] = vn - vm1[e];
                 s_d[0][e][c] = Dx[e];
+                vm2[e] = vm1[e];
                 vm1[e] = v1[e];
                 v1[e] = vn;
                 s_vr[(i + 1) & 1][e][c] = vn;
             }
             ri1 = ri;
-            s_mu[vc][c] = mu_c;
+            mu_sum = (float)(0.5 * (mu_c + mu_prev));   // src/fluxes.c:191
+            s_mu[c] = mu_sum;
         }
         lds_barrier();
 
@@ -286,8 +292,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             }
             float R, vel[3], Pf;
             mx::riemann<0>(l, r, R, vel, Pf);
-            const float vf[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
-            const float vbk[3] = {s_visc[vb][0][c], s_visc[vb][1][c], s_visc[vb][2][c]};
+            // this cell (plane i) and its x neighbour (plane i - 1)
+            const float vf[3] = {vm1[0], vm1[1], vm1[2]};
+            const float vbk[3] = {vm2[0], vm2[1], vm2[2]};
             float gr[3][3];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -296,10 +303,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 gr[q][2] = (Dz[q] + Dz_prev[q]) * r4[2];
             }
             float mu_dyn = 0.f;
-            if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c] + s_mu[vb][c]));
+            if (a.ph.dynamic) mu_dyn = mu_sum;
             float Fx[5];
-            mx::face_flux<0>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c], s_visc[vb][3][c], s_visc[vc][4][c],
-                             s_visc[vb][4][c], gr, mu_dyn, Fx);
+            mx::face_flux<0>(a.ph, R, vel, Pf, vf, vbk, Tcur, Tprev, Rcur, Rprev, gr, mu_dyn, Fx);
             if (i - 1 >= i0) {
                 const long long po = (long long)(i - 1) * si;
                 double ov[5];
@@ -325,6 +331,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
             for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
         }
         xp_prev = xp;
+        Tprev = Tcur;
+        Rprev = Rcur;
+        mu_prev = mu_c;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             Dy_prev[q] = Dy[q];
@@ -332,8 +341,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         }
 
         if (inseg) {
-            const float vbk[3] = {s_visc[vc][0][c], s_visc[vc][1][c], s_visc[vc][2][c]};
-            const float Tb = s_visc[vc][3][c], Rb = s_visc[vc][4][c];
+            const float vbk[3] = {vm1[0], vm1[1], vm1[2]};
+            const float Tb = Tcur, Rb = Rcur;
             if (yface) {
                 mx::FaceF up;   // - state of the face above: the neighbour's
 #pragma unroll
@@ -354,7 +363,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 }
                 float R, vel[3], Pf;
                 mx::riemann<1>(l, r, R, vel, Pf);
-                const float vf[3] = {s_visc[vc][0][c + BK], s_visc[vc][1][c + BK], s_visc[vc][2][c + BK]};
+                const float vf[3] = {s_visc[0][c + BK], s_visc[1][c + BK], s_visc[2][c + BK]};
                 float gr[3][3];
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
@@ -363,9 +372,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     gr[q][2] = (s_d[2][q][c + BK] + Dz[q]) * r4[2];
                 }
                 float mu_dyn = 0.f;   // cell (i, j + 1) and its x neighbour (src/fluxes.c:360)
-                if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + BK] + s_mu[vb][c + BK]));
+                if (a.ph.dynamic) mu_dyn = s_mu[c + BK];
                 float Fy[5];
-                mx::face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + BK], Tb, s_visc[vc][4][c + BK], Rb,
+                mx::face_flux<1>(a.ph, R, vel, Pf, vf, vbk, s_visc[3][c + BK], Tb, s_visc[4][c + BK], Rb,
                                  gr, mu_dyn, Fy);
 #pragma unroll
                 for (int f = 0; f < 5; f++) s_fy[f][c] = Fy[f];
@@ -378,7 +387,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                 r.E = s_zmE[c + 1];
                 float R, vel[3], Pf;
                 mx::riemann<2>(l, r, R, vel, Pf);
-                const float vf[3] = {s_visc[vc][0][c + 1], s_visc[vc][1][c + 1], s_visc[vc][2][c + 1]};
+                const float vf[3] = {s_visc[0][c + 1], s_visc[1][c + 1], s_visc[2][c + 1]};
                 float gr[3][3];
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
@@ -387,9 +396,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
                     gr[q][1] = (s_d[1][q][c + 1] + Dy[q]) * r4[1];
                 }
                 float mu_dyn = 0.f;   // cell (i, k + 1) and its x neighbour (src/fluxes.c:531)
-                if (a.ph.dynamic) mu_dyn = (float)(0.5 * (s_mu[vc][c + 1] + s_mu[vb][c + 1]));
+                if (a.ph.dynamic) mu_dyn = s_mu[c + 1];
                 float Fz[5];
-                mx::face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[vc][3][c + 1], Tb, s_visc[vc][4][c + 1], Rb, gr,
+                mx::face_flux<2>(a.ph, R, vel, Pf, vf, vbk, s_visc[3][c + 1], Tb, s_visc[4][c + 1], Rb, gr,
                                  mu_dyn, Fz);
 #pragma unroll
                 for (int f = 0; f < 5; f++) s_fz[f][c] = Fz[f];
