@@ -252,9 +252,9 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
     c->g.sj = D + 2;
     c->g.si = (long long)(H + 2) * (D + 2);
     c->g.G = (long long)(L + 2) * c->g.si;
-    if (c->cfg.sgs_model == FSL_SGS_LEARNED && (long long)L * H * D >= 0x80000000LL) {
-        // the network kernel decodes interior cells from a 32-bit index
-        fprintf(stderr, "fsl_create: %d x %d x %d cells exceed the learned closure's 2^31 cells\n", L, H, D);
+    if (c->cfg.sgs_model == FSL_SGS_LEARNED && c->g.si * 8 > 0x7fffffffLL) {
+        // the network kernel addresses one (j, k) plane of a field with 32-bit byte offsets
+        fprintf(stderr, "fsl_create: a %d x %d plane exceeds the learned closure's 2^31-byte planes\n", H, D);
         delete c;
         return FSL_ERR_UNSUPPORTED;
     }

@@ -1,8 +1,8 @@
 """Size limits of fsl_create, checked before any device call (runs without a
-GPU): the mixed-precision stage kernel addresses one (j, k) plane of a field
-with 32-bit byte offsets, the learned closure decodes interior cells from a
-31-bit index; beyond them fsl_create returns FSL_ERR_UNSUPPORTED.  A mesh
-with an empty direction is FSL_ERR_ARG."""
+GPU): the mixed-precision stage kernel and the learned closure's network
+kernel address one (j, k) plane of a field with 32-bit byte offsets; beyond
+that fsl_create returns FSL_ERR_UNSUPPORTED.  A mesh with an empty direction
+is FSL_ERR_ARG."""
 import pytest
 
 import freeshearlayer_amd as fsl
@@ -14,8 +14,8 @@ def test_mixed_path_rejects_a_plane_beyond_32_bit_offsets():
         fsl.Context(case, fsl.config(precision=fsl.FSL_PREC_MIXED, fused=1))
 
 
-def test_learned_closure_rejects_more_than_2_31_cells():
-    case = fsl.case_reference(2048, 1024, 1024)    # 2^31 interior cells
+def test_learned_closure_rejects_a_plane_beyond_32_bit_offsets():
+    case = fsl.case_reference(4, 16384, 16384)     # (H + 2)(D + 2) * 8 bytes > 2^31 - 1
     with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
         fsl.Context(case, fsl.config(sgs_model=fsl.FSL_SGS_LEARNED, fused=1))
 
