@@ -3,8 +3,8 @@
 // Each function restates one piece of the reference's arithmetic in the
 // reference's association order, so that with T = double and FP contraction
 // off the results are bit-identical to src/ compiled with real = double.
-// The FP32 (mixed-precision) kernels instantiate the same templates with
-// T = float.
+// FP32 kernels may instantiate the templates with T = float (Divisor and
+// root are then plain division and sqrt).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -96,6 +96,73 @@ __device__ __forceinline__ T minmod(T x, T y)
     return (x * y <= T(0)) ? T(0) : __builtin_copysign(m, x);
 }
 
+// Quotients over one denominator.  For T = float: plain division.  For
+// T = double: the compiler's correctly rounded n / d is v_div_scale of both
+// operands, v_rcp of d, two Newton steps on the reciprocal, m = n r, one
+// residual step (v_div_fmas) and v_div_fixup.  The reciprocal and its Newton
+// steps depend on d alone: Divisor<double> does them once and div() does the
+// rest per numerator, with the same operations on the same values.  The two
+// steps left out change nothing in the normal range: v_div_scale rescales
+// only subnormal or near-overflow operands or quotients (|d| outside
+// [2^-1022, 2^1022), 0 < |n| < 2^-969, quotients outside the normal range),
+// and for finite non-zero operands v_div_fixup only returns the signed zero
+// of a zero numerator, which copysign(q, m) gives (m carries the quotient's
+// sign).  Densities, momenta, pressures and their differences in the flow lie
+// far inside that range, where div() is n / d bit for bit.
+template <class T>
+struct Divisor {
+    T d;
+    __device__ __forceinline__ explicit Divisor(T den) : d(den) {}
+    __device__ __forceinline__ T div(T n) const { return n / d; }
+};
+
+template <>
+struct Divisor<double> {
+    double d, r;
+    __device__ __forceinline__ explicit Divisor(double den) : d(den)
+    {
+        r = __builtin_amdgcn_rcp(den);
+        double e = __builtin_fma(-den, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-den, r, 1.0);
+        r = __builtin_fma(r, e, r);
+    }
+    __device__ __forceinline__ double div(double n) const
+    {
+        const double m = n * r;
+        const double rem = __builtin_fma(-d, m, n);
+        return __builtin_copysign(__builtin_fma(rem, r, m), m);
+    }
+};
+
+template <class T>
+__device__ __forceinline__ T divide(T n, T d) { return Divisor<T>(d).div(n); }
+
+// Square root.  For T = double: the compiler's correctly rounded sqrt is a
+// pre-scaling of arguments below 2^-767 by 2^256, v_rsq, Goldschmidt steps on
+// g = x y and h = y / 2 with two residual corrections, the matching rescale
+// of the root and v_cmp_class passing +-0 and +inf through.  root() keeps
+// every step but the scaling, which is an identity from 2^-767 up, so for
+// those arguments and for zero, infinity, negatives and NaN it is sqrt(x) bit
+// for bit.  (Squared sound speeds of the flow are of order 10^5.)
+template <class T>
+__device__ __forceinline__ T root(T x) { return sqrt(x); }
+
+template <>
+__device__ __forceinline__ double root(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;   // -0, +0, +inf
+}
+
 // Per-cell quantities of the characteristic transform,
 // src/reconstruction.c:64-88, 104, 142-145
 template <class T>
@@ -118,11 +185,11 @@ __device__ __forceinline__ void cell_prims_rv(const T q[5], T R, const T vel[3],
     c.vel[1] = vel[1];
     c.vel[2] = vel[2];
     c.P = (q[4] - T(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * c.R) * T(kK_1);
-    c.C = sqrt(T(kK) * c.P * c.R);
+    c.C = root(T(kK) * c.P * c.R);
     T ff = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
     c.aa = T(kK_1_2) * ff;
-    c.ll = T(1.) / (c.C + c.C);
-    c.ee = c.ll / c.C;
+    c.ll = divide(T(1.), c.C + c.C);
+    c.ee = divide(c.ll, c.C);
     c.gg = -ff * c.ee;
     c.hh = -(c.ee + c.ee);
     c.S11 = c.aa - c.C * c.C;
@@ -137,7 +204,7 @@ __device__ __forceinline__ void cell_prims_rv(const T q[5], T R, const T vel[3],
 template <class T>
 __device__ __forceinline__ void cell_prims(const T q[5], Prim<T>& c)
 {
-    const T R = T(1.) / q[0];
+    const T R = divide(T(1.), q[0]);
     const T vel[3] = {q[1] * R, q[2] * R, q[3] * R};
     cell_prims_rv(q, R, vel, c);
 }
@@ -219,18 +286,19 @@ __device__ __forceinline__ void riemann(const T l[5], const T r[5], T& Ro, T vel
 {
     T lv[3], rv[3];
     const T lR = l[0], rR = r[0];
-    lv[0] = l[1] / lR; lv[1] = l[2] / lR; lv[2] = l[3] / lR;
+    const Divisor<T> byL(lR), byR(rR);   // four quotients over each density
+    lv[0] = byL.div(l[1]); lv[1] = byL.div(l[2]); lv[2] = byL.div(l[3]);
     const T lP = (l[4] - T(0.5) * lR * (lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2])) * T(kK_1);
-    const T lC = sqrt(T(kK) * lP / lR);
+    const T lC = root(byL.div(T(kK) * lP));
     const T ljo = T(kNK_1) * lP;
-    T RU = lP / (lR * lC);
+    T RU = divide(lP, lR * lC);
     const T ljp = lv[N] + RU;
     const T ljm = lv[N] - RU;
-    rv[0] = r[1] / rR; rv[1] = r[2] / rR; rv[2] = r[3] / rR;
+    rv[0] = byR.div(r[1]); rv[1] = byR.div(r[2]); rv[2] = byR.div(r[3]);
     const T rP = (r[4] - T(0.5) * rR * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2])) * T(kK_1);
-    const T rC = sqrt(T(kK) * rP / rR);
+    const T rC = root(byR.div(T(kK) * rP));
     const T rjo = T(kNK_1) * rP;
-    RU = rP / (rR * rC);
+    RU = divide(rP, rR * rC);
     const T rjp = rv[N] + RU;
     const T rjm = rv[N] - RU;
     T Un = T(0.5) * (lv[N] + rv[N]);
@@ -240,16 +308,16 @@ __device__ __forceinline__ void riemann(const T l[5], const T r[5], T& Ro, T vel
     // operands are chosen first and divided once (the same operations on the
     // same values as the reference's branches, without evaluating both)
     const bool sp = C_p > T(0.), sm = C_m > T(0.), so = C_o > T(0.);
-    const T al_p = T(kK_1_2) * (sp ? ljm - ljp : rjm - rjp) / (sp ? ljo : rjo);
+    const T al_p = divide(T(kK_1_2) * (sp ? ljm - ljp : rjm - rjp), sp ? ljo : rjo);
     const T J_p = sp ? ljp : rjp;
-    const T al_m = T(kK_1_2) * (sm ? ljp - ljm : rjp - rjm) / (sm ? ljo : rjo);
+    const T al_m = divide(T(kK_1_2) * (sm ? ljp - ljm : rjp - rjm), sm ? ljo : rjo);
     const T J_m = sm ? ljm : rjm;
     const T a = T(kH05K) * (so ? ljp - ljm : rjp - rjm);
     const T al_o = -a * a;
     const T J_o = so ? ljo : rjo;
-    const T P = (J_p - J_m) / (al_p - al_m);
+    const T P = divide(J_p - J_m, al_p - al_m);
     Un = J_p - al_p * P;
-    Ro = (J_o - P) / al_o;
+    Ro = divide(J_o - P, al_o);
     const bool left = Un > T(0);
 #pragma unroll
     for (int t = 0; t < 3; t++) vel[t] = left ? lv[t] : rv[t];
@@ -262,7 +330,7 @@ __device__ __forceinline__ void riemann(const T l[5], const T r[5], T& Ro, T vel
 template <class T>
 __device__ __forceinline__ void vel_recip(T r, T m0, T m1, T m2, T v[3], T* rr = nullptr)
 {
-    const T RU = T(1.) / r;
+    const T RU = divide(T(1.), r);
     v[0] = m0 * RU; v[1] = m1 * RU; v[2] = m2 * RU;
     if (rr) *rr = RU;
 }
@@ -271,9 +339,10 @@ __device__ __forceinline__ void vel_recip(T r, T m0, T m1, T m2, T v[3], T* rr =
 template <class T>
 __device__ __forceinline__ void vel_div_T(T r, T m0, T m1, T m2, T e, T v[3], T& Tc)
 {
-    v[0] = m0 / r; v[1] = m1 / r; v[2] = m2 / r;
+    const Divisor<T> byR(r);
+    v[0] = byR.div(m0); v[1] = byR.div(m1); v[2] = byR.div(m2);
     const T P = (e - T(0.5) * r * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])) * T(kK_1);
-    Tc = P / (T(kRair) * r);
+    Tc = divide(P, T(kRair) * r);
 }
 
 // Viscous + SGS part and flux assembly of one face, src/fluxes.c:171-220
