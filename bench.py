@@ -23,7 +23,8 @@ Metric: Mcell-updates/s per RK stage = cells x stages x steps / seconds.
 
 Prints one JSON line.  `value` is the headline path's throughput;
 `fp64_dynamic` the comparison leg and `speedup_vs_fp64_dynamic` their
-ratio.  `roofline` prices the dominant kernel's algorithmic bytes per
+ratio; `fp64_fast_dynamic` the same FP64 leg with the fast build of the stage
+kernel (FMA contraction and reciprocal quotients, not bit-identical).  `roofline` prices the dominant kernel's algorithmic bytes per
 launch against its average launch duration, measured with HIP events on the
 context's stream over the timed steps; `mlp_roofline` the learned-closure
 kernel's algorithmic FLOPs against the FP16 dense peak (its products are FP16).  `traffic` (HBM bytes per
@@ -36,7 +37,7 @@ not built, the FP64 CPU oracle (the same algorithm restated in C; also
 reported as `cpu_port`) -- a reported baseline, not the target.
 
     python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--mesh 384x256x256]
-                    [--sgs learned|dynamic|constant] [--precision mixed|fp64]
+                    [--sgs learned|dynamic|constant] [--precision mixed|fp64|fp64fast]
                     [--mach 0.4] [--path fused|unfused] [--model file] [--no-compare] [--no-cpu]
 """
 import argparse
@@ -110,7 +111,8 @@ def workload_name(precision, sgs, path, L, H, D, mach, activation=None):
     net = " (%s network)" % {fsl.FSL_MLP_TANH: "tanh", fsl.FSL_MLP_RELU: "ReLU"}[activation] \
         if sgs == "learned" and activation is not None else ""
     return "%s %s-SGS%s %s path, %dx%dx%d cells, Mc %.1f, 3-stage TVD RK" % (
-        "FP32+FP64" if precision == "mixed" else "FP64", sgs, net, path, L, H, D, mach)
+        {"mixed": "FP32+FP64", "fp64": "FP64", "fp64fast": "FP64 (FMA build)"}[precision], sgs, net, path,
+        L, H, D, mach)
 
 
 def spin_up(ctx, cap):
@@ -174,6 +176,7 @@ def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=Tr
 KERNEL_SOURCES = {
     ("fused_stage", "mixed"): ["fsl_fused_mixed.hip", "fsl_device_mixed.hpp", "fsl_device.hpp", "fsl_buffer.hpp"],
     ("fused_stage", "fp64"): ["fsl_fused.hip", "fsl_device.hpp", "fsl_buffer.hpp"],
+    ("fused_stage", "fp64fast"): ["fsl_fused_fast.hip", "fsl_fused.hip", "fsl_device.hpp", "fsl_buffer.hpp"],
     ("mlp_mu", "mixed"): ["fsl_mlp.hip"],
     ("mlp_mu", "fp64"): ["fsl_mlp.hip"],
 }
@@ -296,7 +299,7 @@ def main(argv=None):
                     help="most untimed steps the learned leg may take to leave the dynamic fallback")
     ap.add_argument("--mesh", default="384x256x256")
     ap.add_argument("--sgs", default="learned", choices=["learned", "dynamic", "constant"])
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64fast"])
     ap.add_argument("--path", default="fused", choices=["fused", "unfused"])
     ap.add_argument("--mach", type=float, default=0.4)
     ap.add_argument("--cfl", type=float, default=0.5)
@@ -352,7 +355,7 @@ def main(argv=None):
             failures.append("no learned-closure kernel launch was timed")
     final_dt = ctx.dt
     ctx.close()
-    compare = None
+    compare = compare_fast = None
     if not a.no_compare and not (a.sgs == "dynamic" and a.precision == "fp64"):
         print("[bench] comparison leg: FP64 dynamic-SGS, same grid", file=sys.stderr, flush=True)
         # as many untimed steps as the headline leg took, so both time the same flow
@@ -363,6 +366,17 @@ def main(argv=None):
         compare = {"value": round(ref["value"], 2), "ms_per_step": round(ref["t"] / a.steps * 1e3, 4),
                    "workload": workload_name("fp64", "dynamic", a.path, L, H, D, a.mach), "roofline": rl}
         ref["ctx"].close()
+        if a.path == "fused":
+            # the fast build of the FP64 stage kernel (FMA contraction, reciprocal
+            # quotients; within 1e-12 of the parity path after one step)
+            print("[bench] comparison leg: FP64 (FMA build) dynamic-SGS, same grid", file=sys.stderr, flush=True)
+            fast = time_leg(L, H, D, "dynamic", "fp64fast", a.path, a.cfl, a.mach, a.steps, a.warmup,
+                            not a.no_profile, presteps=leg["spinup_steps"])
+            fwl = workload_name("fp64fast", "dynamic", a.path, L, H, D, a.mach)
+            frl, _ = roofline_of(fast["prof"], L, H, D, "dynamic", "fp64fast", fwl)
+            compare_fast = {"value": round(fast["value"], 2), "ms_per_step": round(fast["t"] / a.steps * 1e3, 4),
+                            "workload": fwl, "roofline": frl}
+            fast["ctx"].close()
     cpu = port = None
     if not a.no_cpu:
         print("[bench] CPU reference sample", file=sys.stderr, flush=True)
@@ -381,7 +395,7 @@ def main(argv=None):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(leg["t"] / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64+f16" if a.sgs == "learned" else "f32+f64",
+        "dtype": "f64" if a.precision.startswith("fp64") else "f32+f64+f16" if a.sgs == "learned" else "f32+f64",
         "data": "synthetic shear layer (tanh profile, Mc %.1f, U2/U1 0.5, Re 500, 2%% seeded perturbations, "
                 "acoustic CFL %.2f)" % (a.mach, a.cfl),
         "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,
@@ -396,6 +410,7 @@ def main(argv=None):
         "fp64_dynamic": compare,
         "speedup_vs_fp64_dynamic": (round(leg["value"] / compare["value"], 3)
                                     if compare and value is not None else None),
+        "fp64_fast_dynamic": compare_fast,
         "spinup_steps": leg["spinup_steps"],
         "learned_window": window,
         "learned_status": leg["status"],

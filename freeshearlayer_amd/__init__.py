@@ -22,7 +22,7 @@ FSL_OK = 0
 FSL_ERR_ARG, FSL_ERR_HIP, FSL_ERR_NOMEM, FSL_ERR_NONFINITE, FSL_ERR_IO, FSL_ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 FSL_SGS_CONSTANT, FSL_SGS_DYNAMIC, FSL_SGS_LEARNED = 0, 1, 2
 FSL_SGS_AVG_NONE, FSL_SGS_AVG_Z, FSL_SGS_AVG_XZ = 0, 1, 2
-FSL_PREC_FP64, FSL_PREC_MIXED = 0, 1
+FSL_PREC_FP64, FSL_PREC_MIXED, FSL_PREC_FP64_FAST = 0, 1, 2
 FSL_CFL_CONVECTIVE, FSL_CFL_ACOUSTIC = 0, 1
 FSL_MLP_TANH, FSL_MLP_RELU = 0, 1
 MLP_ACTIVATIONS = {"tanh": FSL_MLP_TANH, "relu": FSL_MLP_RELU}

@@ -29,7 +29,8 @@ ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
-# -ffp-contract=off keeps the FP64 path in the reference's rounding order;
+# -ffp-contract=off keeps the FP64 path in the reference's rounding order
+# (except in the fast build named below);
 # packed FP32 instructions are disabled (the FP32 kernels use scalar VALU).
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -38,8 +39,10 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-std=c99", "-Wall", "-fPIC", "-ffp-contract=off", "-I" + INCLUDE]
 
-HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_mixed.hip",
-               "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip", "fsl_sgs_march.hip"]
+HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip", "fsl_fused_fast.hip",
+               "fsl_fused_mixed.hip", "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip", "fsl_sgs_march.hip"]
+# the fast build of the FP64 stage kernel (FSL_PREC_FP64_FAST) contracts a * b + c into FMA
+CONTRACTED = {"fsl_fused_fast.hip"}
 HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c", "fsl_files.c"]
 
 
@@ -75,8 +78,11 @@ def build_native(jobs=8, force=True, verbose=False):
     for s in HIP_SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(BUILD, s.replace(".hip", ".o"))
-        if force or not _newer(obj, [src] + hdrs):
-            tasks.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        deps = [src] + hdrs + ([os.path.join(CSRC, "fsl_fused.hip")] if s in CONTRACTED else [])
+        if force or not _newer(obj, deps):
+            flags = [("-ffp-contract=fast" if f == "-ffp-contract=off" else f) for f in HIP_FLAGS] \
+                if s in CONTRACTED else HIP_FLAGS
+            tasks.append([HIPCC] + flags + ["-c", src, "-o", obj])
     for s in HOST_SOURCES + ["fsl_layer2_api.c", "fsl_layer2_main.c", "fsl_layer2_p.c"]:
         src = os.path.join(HOST, s)
         obj = os.path.join(BUILD, s.replace(".c", ".o"))

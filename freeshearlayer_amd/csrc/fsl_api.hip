@@ -229,10 +229,18 @@ int fsl_create(const fsl_case* cs, const fsl_config* cfg, fsl_ctx** out)
     c->cs = *cs;
     if (cfg) c->cfg = *cfg;
     else fsl_config_defaults(&c->cfg);
-    if ((c->cfg.precision != FSL_PREC_FP64 && c->cfg.precision != FSL_PREC_MIXED) ||
+    if ((c->cfg.precision != FSL_PREC_FP64 && c->cfg.precision != FSL_PREC_MIXED &&
+         c->cfg.precision != FSL_PREC_FP64_FAST) ||
         c->cfg.sgs_model < FSL_SGS_CONSTANT || c->cfg.sgs_model > FSL_SGS_LEARNED) {
         delete c;
         return FSL_ERR_ARG;
+    }
+    if (c->cfg.precision == FSL_PREC_FP64_FAST && !c->cfg.fused) {
+        // the fast build is one of the fused stage kernel; the unfused entry
+        // points are the parity kernels only
+        fprintf(stderr, "fsl_create: precision FSL_PREC_FP64_FAST needs the fused path (fused = 1)\n");
+        delete c;
+        return FSL_ERR_UNSUPPORTED;
     }
     if (c->cfg.fused && cs->nStages != 3) {
         // the fused stage kernels implement the three-stage TVD RK only; the

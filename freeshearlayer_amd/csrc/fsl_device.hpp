@@ -108,7 +108,8 @@ __device__ __forceinline__ T minmod(T x, T y)
 // and for finite non-zero operands v_div_fixup only returns the signed zero
 // of a zero numerator, which copysign(q, m) gives (m carries the quotient's
 // sign).  Densities, momenta, pressures and their differences in the flow lie
-// far inside that range, where div() is n / d bit for bit.
+// far inside that range, where div() is n / d bit for bit.  The fast build of
+// the FP64 stage kernel (FSL_FP64_FAST) takes n r alone, within an ulp or so.
 template <class T>
 struct Divisor {
     T d;
@@ -129,9 +130,13 @@ struct Divisor<double> {
     }
     __device__ __forceinline__ double div(double n) const
     {
+#ifdef FSL_FP64_FAST
+        return n * r;
+#else
         const double m = n * r;
         const double rem = __builtin_fma(-d, m, n);
         return __builtin_copysign(__builtin_fma(rem, r, m), m);
+#endif
     }
 };
 

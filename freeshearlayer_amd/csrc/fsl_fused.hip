@@ -352,6 +352,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
     if (STAGE == 3 && a.cfl_out) cfl_fold(cfl_m, a.cfl_out);
 }
 
+#ifndef FSL_FP64_FAST   // (shared with the fast build, which links to these)
 // Segments along x.  Workgroups of one launch run in rounds of `resident`
 // (per_cu per CU); a round takes about (segment length + 2) plane steps
 // (the march starts two planes early).  Pick the count that minimises
@@ -367,9 +368,11 @@ int device_cus()
     }
     return n;
 }
+#endif
 
 }  // namespace
 
+#ifndef FSL_FP64_FAST
 int fused_segments(int L, int cols, int per_cu, int* seg_len)
 {
     const long long resident = (long long)per_cu * device_cus();
@@ -390,6 +393,7 @@ int fused_segments(int L, int cols, int per_cu, int* seg_len)
     *seg_len = best_len;
     return best;
 }
+#endif
 
 namespace {
 
@@ -447,6 +451,50 @@ int fused_variant()
 
 }  // namespace
 
+// The FP64 launch of one stage.  Built twice: as the parity kernel
+// (launch_fused_fp64) and, with FSL_FP64_FAST defined and FMA contraction
+// enabled, as the fast build of the same source (launch_fused_fp64_fast).
+#ifdef FSL_FP64_FAST
+void launch_fused_fp64_fast(const CtxView& v, int stage, const double* in, double* out)
+#else
+void launch_fused_fp64(const CtxView& v, int stage, const double* in, double* out)
+#endif
+{
+    FusedArgs a{};
+    for (int f = 0; f < 5; f++) {
+        a.in.f[f] = in + f * v.g.G;
+        a.base.f[f] = v.U + f * v.g.G;
+        a.out.f[f] = out + f * v.g.G;
+    }
+    a.mu = v.mu;
+    a.g = v.g;
+    a.ph = v.ph;
+    a.b = v.bnd;
+    a.dtX = v.dtX;
+    a.dtY = v.dtY;
+    a.dtZ = v.dtZ;
+    // stage 3 also reduces the CFL quantity of the new state for the next
+    // step's time step (fsl_check_co_num uses it while U and dt are unchanged)
+    a.cfl_out = stage == 3 ? v.red : nullptr;
+    a.cfl_zero = stage == 2 ? v.red : nullptr;
+    a.cfl_mode = v.cfl_mode;
+    // one buffer resource per state plane where the five fields fit 32-bit
+    // offsets, (4 G + one plane) * 8 < 2^32 bytes; otherwise one per field
+    const unsigned long long span = (4ull * (unsigned long long)v.g.G + (unsigned long long)v.g.si) * 8ull;
+    const char* e = getenv("FSL_FP64_FIELD_RSRC");
+    const bool pk = span <= 0xffffffffull && !(e && atoi(e) == 1);
+    a.fb = (unsigned)(pk ? v.g.G * 8 : 0);
+    if (pk) {
+        switch (fused_variant()) {
+        case 1: launch_variant<16, 16, 1, true>(v.stream, stage, a, 1); break;
+        default: launch_variant<16, 16, 2, true>(v.stream, stage, a, 2); break;
+        }
+    } else {
+        launch_variant<16, 16, 2, false>(v.stream, stage, a, 2);
+    }
+}
+
+#ifndef FSL_FP64_FAST
 int fused_stage(fsl_ctx* ctx, int stage)
 {
     CtxView v = ctx_view(ctx);
@@ -467,44 +515,11 @@ int fused_stage(fsl_ctx* ctx, int stage)
         if (rc) return rc;
     }
     v = ctx_view(ctx);   // the learned closure may have swapped the mu_SGS array
-    FusedArgs a{};
-    for (int f = 0; f < 5; f++) {
-        a.in.f[f] = in + f * v.g.G;
-        a.base.f[f] = v.U + f * v.g.G;
-        a.out.f[f] = out + f * v.g.G;
-    }
-    a.mu = v.mu;
-    a.g = v.g;
-    a.ph = v.ph;
-    a.b = v.bnd;
-    a.dtX = v.dtX;
-    a.dtY = v.dtY;
-    a.dtZ = v.dtZ;
-    // stage 3 also reduces the CFL quantity of the new state for the next
-    // step's time step (fsl_check_co_num uses it while U and dt are unchanged)
-    a.cfl_out = stage == 3 ? v.red : nullptr;
-    a.cfl_zero = stage == 2 ? v.red : nullptr;
-    a.cfl_mode = v.cfl_mode;
     if (stage == 3 && (rc = ctx_cfl_reset(ctx))) return rc;
     ctx_prof(ctx, K_FUSED_XYZ, 0);
-    if (v.precision == FSL_PREC_MIXED) {
-        launch_fused_mixed(v, stage, in, out);
-    } else {
-        // one buffer resource per state plane where the five fields fit 32-bit
-        // offsets, (4 G + one plane) * 8 < 2^32 bytes; otherwise one per field
-        const unsigned long long span = (4ull * (unsigned long long)v.g.G + (unsigned long long)v.g.si) * 8ull;
-        const char* e = getenv("FSL_FP64_FIELD_RSRC");
-        const bool pk = span <= 0xffffffffull && !(e && atoi(e) == 1);
-        a.fb = (unsigned)(pk ? v.g.G * 8 : 0);
-        if (pk) {
-            switch (fused_variant()) {
-            case 1: launch_variant<16, 16, 1, true>(v.stream, stage, a, 1); break;
-            default: launch_variant<16, 16, 2, true>(v.stream, stage, a, 2); break;
-            }
-        } else {
-            launch_variant<16, 16, 2, false>(v.stream, stage, a, 2);
-        }
-    }
+    if (v.precision == FSL_PREC_MIXED) launch_fused_mixed(v, stage, in, out);
+    else if (v.precision == FSL_PREC_FP64_FAST) launch_fused_fp64_fast(v, stage, in, out);
+    else launch_fused_fp64(v, stage, in, out);
     ctx_prof(ctx, K_FUSED_XYZ, 1);
     rc = ctx_check(ctx, "fused stage");
     if (rc) return rc;
@@ -514,5 +529,6 @@ int fused_stage(fsl_ctx* ctx, int stage)
     ctx_set_cur(ctx, stage == 3 ? 0 : 1);
     return FSL_OK;
 }
+#endif
 
 }  // namespace fsl

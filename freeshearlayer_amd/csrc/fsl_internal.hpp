@@ -150,5 +150,7 @@ struct CtxView {
     int cfl_mode;
 };
 void launch_fused_mixed(const CtxView& v, int stage, const double* in, double* out);
+void launch_fused_fp64(const CtxView& v, int stage, const double* in, double* out);
+void launch_fused_fp64_fast(const CtxView& v, int stage, const double* in, double* out);
 
 }  // namespace fsl

@@ -11,7 +11,7 @@ import numpy as np
 import freeshearlayer_amd as fsl
 
 SGS = {"constant": fsl.FSL_SGS_CONSTANT, "dynamic": fsl.FSL_SGS_DYNAMIC, "learned": fsl.FSL_SGS_LEARNED}
-PRECISION = {"fp64": fsl.FSL_PREC_FP64, "mixed": fsl.FSL_PREC_MIXED}
+PRECISION = {"fp64": fsl.FSL_PREC_FP64, "mixed": fsl.FSL_PREC_MIXED, "fp64fast": fsl.FSL_PREC_FP64_FAST}
 
 
 def parse_mesh(s):

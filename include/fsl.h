@@ -155,6 +155,10 @@ typedef struct fsl_ctx fsl_ctx;
 
 #define FSL_PREC_FP64    0   /* parity path: FP64, no contraction, reference op order */
 #define FSL_PREC_MIXED   1   /* FP32 stencils and SGS, FP64 state / RK / p recovery / CFL */
+#define FSL_PREC_FP64_FAST 2 /* FP64 with the fused stage kernel built with FMA contraction
+                                and reciprocal quotients (needs fused = 1); within 1e-12
+                                relative of the parity path after one step, not bit-identical;
+                                every other kernel is the parity path's */
 
 #define FSL_CFL_CONVECTIVE 0 /* src/helpers.c:23-52: max (rho u / rho) dt/dx, rescale */
 #define FSL_CFL_ACOUSTIC   1 /* dt = cfl / max[(|u|+c)/dx + (|v|+c)/dy + (|w|+c)/dz] */

@@ -46,3 +46,21 @@ def test_fused_path_rejects_two_stages():
     for n in (1, 4):   # not a stage count of the reference (src/evolution.c:15-27)
         with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_ARG):
             fsl.Context(fsl.case_reference(16, 12, 8, nStages=n), fsl.config(fused=0))
+
+
+def test_fast_fp64_needs_the_fused_path():
+    """FSL_PREC_FP64_FAST is a build of the fused stage kernel: an unfused
+    context with it is refused before any device call."""
+    case = fsl.case_reference(16, 12, 8)
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_UNSUPPORTED):
+        fsl.Context(case, fsl.config(precision=fsl.FSL_PREC_FP64_FAST, fused=0))
+    with pytest.raises(fsl.FslError, match=r"\(%d\)" % fsl.FSL_ERR_ARG):
+        fsl.Context(case, fsl.config(precision=3, fused=1))
+
+
+def test_precision_codes_match_the_header():
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "fsl.h")).read()
+    codes = dict((m.group(1), int(m.group(2))) for m in re.finditer(r"#define (FSL_PREC_\w+)\s+(\d+)", hdr))
+    assert len(codes) == 3 and all(getattr(fsl, k) == v for k, v in codes.items()), codes

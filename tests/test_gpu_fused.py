@@ -1,7 +1,9 @@
 """GPU parity of the fused stage path: one kernel per stage (plus the ghost
 fill and, once per step, the dynamic model) against the CPU oracle, bit for
 bit, on meshes that do and do not divide the 14 x 14 workgroup tile, and
-against the unfused device path at the benchmark size."""
+against the unfused device path at the benchmark size; the fast build of the
+FP64 stage kernel (FSL_PREC_FP64_FAST) within 1e-12 of the oracle after one
+step, and its divergence over 100 steps."""
 import numpy as np
 import pytest
 
@@ -128,3 +130,84 @@ def test_fused_matches_oracle_at_192x128x96():
             assert_bitwise(orc.state()[:, 1:-1, 1:-1, 1:-1], ctx.download_state()[:, 1:-1, 1:-1, 1:-1],
                            "interior U after step %d" % n)
     orc.close()
+
+
+def _shear_oracle(L, H, D, dynamic):
+    from oracle_lib import Oracle
+    from freeshearlayer_amd import shear_case
+    case, U, u_in, shear = shear_case.initial(L, H, D)
+    orc = Oracle(L, H, D, dynamic=dynamic)
+    orc.set("mu_L", case.mu_L)
+    orc.set("BL_HIG", case.BL_HIG)
+    orc.u_in()[:] = u_in
+    orc.derive()
+    orc.set_state(U)
+    orc.set("X", 0.317391239817)
+    return case, U, u_in, orc
+
+
+def _rel_divergence(a, b):
+    """max |a - b| / max |b| per conserved field (interior cells)"""
+    return [float(np.abs(a[f] - b[f]).max() / np.abs(b[f]).max()) for f in range(5)]
+
+
+@pytest.mark.parametrize("mesh,dynamic", [((64, 48, 32), 0), ((192, 128, 96), 1)])
+def test_fast_fp64_one_step_within_1e12_of_oracle(mesh, dynamic):
+    """The fast build of the FP64 stage kernel (FSL_PREC_FP64_FAST: FMA
+    contraction, quotients by the refined reciprocal) against the CPU oracle
+    after one step on the shear layer: 64 x 48 x 32 with constant Cs and
+    192 x 128 x 96 with the dynamic model.  Tolerance: 1e-12 relative in the
+    max norm of each conserved field.  The time step of the first step comes
+    from the initial state (the same kernel in both builds) and is equal."""
+    L, H, D = mesh
+    case, U, u_in, orc = _shear_oracle(L, H, D, dynamic)
+    cfg = fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC if dynamic else fsl.FSL_SGS_CONSTANT, fused=1,
+                     precision=fsl.FSL_PREC_FP64_FAST)
+    with fsl.Context(case, cfg) as ctx:
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        ctx.set_lcg(0.317391239817, 0)
+        ctx.step()
+        orc.step()
+        assert ctx.dt == orc.get("dt")
+        a, b = ctx.download_state()[:, 1:-1, 1:-1, 1:-1], orc.state()[:, 1:-1, 1:-1, 1:-1]
+    orc.close()
+    div = _rel_divergence(a, b)
+    print("fast FP64, one step, %dx%dx%d: divergence per field %s" % (L, H, D, " ".join("%.1e" % d for d in div)))
+    assert np.all(np.isfinite(a))
+    assert max(div) <= 1e-12, div
+    assert max(div) > 0.0, "the fast build is expected to round differently somewhere"
+
+
+def test_fast_fp64_hundred_step_divergence():
+    """Divergence of the fast FP64 build from the parity path (itself equal
+    to the oracle bit for bit over these 100 steps) on the 64 x 48 x 32 shear
+    layer with the dynamic model, reported at steps 1, 10, 50 and 100; it
+    grows from rounding level and stays below 1e-10 relative here."""
+    from freeshearlayer_amd import shear_case
+    L, H, D = 64, 48, 32
+    case, U, u_in, shear = shear_case.initial(L, H, D)
+    ctxs = []
+    for prec in (fsl.FSL_PREC_FP64, fsl.FSL_PREC_FP64_FAST):
+        ctx = fsl.Context(case, fsl.config(sgs_model=fsl.FSL_SGS_DYNAMIC, fused=1, precision=prec))
+        ctx.set_inflow(u_in)
+        ctx.upload_state(U)
+        ctx.set_lcg(0.317391239817, 0)
+        ctxs.append(ctx)
+    worst = []
+    try:
+        for n in range(1, 101):
+            for ctx in ctxs:
+                ctx.step()
+            if n in (1, 10, 50, 100):
+                b, a = (c.download_state()[:, 1:-1, 1:-1, 1:-1] for c in ctxs)
+                div = _rel_divergence(a, b)
+                worst.append(max(div))
+                print("step %3d: dt %.9e / %.9e, divergence per field %s" % (
+                    n, ctxs[0].dt, ctxs[1].dt, " ".join("%.1e" % d for d in div)))
+                assert np.all(np.isfinite(a))
+                assert abs(ctxs[1].dt - ctxs[0].dt) <= 1e-10 * ctxs[0].dt
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    assert worst[0] <= 1e-12 and worst[-1] <= 1e-10, worst
