@@ -46,6 +46,10 @@ int ctx_ghost(fsl_ctx* c, const double* state, bool z = true);
 void ctx_set_cur(fsl_ctx* c, int is_p);
 int ctx_err(fsl_ctx* c, int code, const char* msg);
 
+#ifdef FSL_FP64_FAST
+#define k_fused k_fused_fast   // the fast build's kernels under a name of their own (profiles)
+#endif
+
 namespace {
 
 struct FusedArgs {
