@@ -62,9 +62,22 @@ def rel_maxnorm(a, b):
 
 
 def assert_bitwise(a, b, what=""):
+    """Equal bit patterns: floating-point arrays are compared as integers of
+    their width, so a signed zero or a NaN payload that differs fails too."""
     a = np.asarray(a); b = np.asarray(b)
-    if not np.array_equal(a, b):
-        d = np.abs(a - b)
-        idx = np.unravel_index(np.nanargmax(d), d.shape)
-        raise AssertionError("%s differs: %d entries, max |diff| %.3e at %s (%.17g vs %.17g)"
-                             % (what, int(np.count_nonzero(a != b)), np.nanmax(d), idx, a[idx], b[idx]))
+    if a.shape != b.shape:
+        raise AssertionError("%s: shapes %s and %s" % (what, a.shape, b.shape))
+    if a.dtype == b.dtype and a.dtype.kind == "f":
+        ui = {2: np.uint16, 4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+        if np.array_equal(a.view(ui), b.view(ui)):
+            return
+        if np.array_equal(a, b, equal_nan=True):
+            n = int(np.count_nonzero(a.view(ui) != b.view(ui)))
+            raise AssertionError("%s: equal values but %d entries differ in the bits of a signed zero or "
+                                 "a NaN" % (what, n))
+    elif np.array_equal(a, b):
+        return
+    d = np.abs(a - b)
+    idx = np.unravel_index(np.nanargmax(d), d.shape)
+    raise AssertionError("%s differs: %d entries, max |diff| %.3e at %s (%.17g vs %.17g)"
+                         % (what, int(np.count_nonzero(a != b)), np.nanmax(d), idx, a[idx], b[idx]))
