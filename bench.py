@@ -171,8 +171,9 @@ def time_leg(L, H, D, sgs, precision, path, cfl, mach, steps, warmup, profile=Tr
     return out
 
 
-# Sources whose text determines each profiled kernel's machine code: a
-# counter figure recorded for other sources is stale and is not reported.
+# Sources whose text determines each profiled kernel's machine code (with
+# build.py, which holds the compile flags): a counter figure recorded for
+# other sources is stale and is not reported.
 KERNEL_SOURCES = {
     ("fused_stage", "mixed"): ["fsl_fused_mixed.hip", "fsl_device_mixed.hpp", "fsl_device.hpp", "fsl_buffer.hpp"],
     ("fused_stage", "fp64"): ["fsl_fused.hip", "fsl_device.hpp", "fsl_buffer.hpp"],
@@ -188,7 +189,7 @@ def sources_sha16(kernel, precision):
     if not files:
         return None
     h = hashlib.sha256()
-    for f in files:
+    for f in files + ["../build.py"]:
         with open(os.path.join(ROOT, "freeshearlayer_amd", "csrc", f), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()[:16]

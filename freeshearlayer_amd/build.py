@@ -43,6 +43,9 @@ HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip",
                "fsl_fused_mixed.hip", "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip", "fsl_sgs_march.hip"]
 # the fast build of the FP64 stage kernel (FSL_PREC_FP64_FAST) contracts a * b + c into FMA
 CONTRACTED = {"fsl_fused_fast.hip"}
+# the scheduler's ILP strategy: 0.8 % faster mixed stage kernel and 1.3 % faster network kernel
+# at 384x256x256 (no spills); the FP64 stage kernel spills with it
+SCHED_ILP = {"fsl_fused_mixed.hip", "fsl_mlp.hip"}
 HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c", "fsl_files.c"]
 
 
@@ -81,7 +84,9 @@ def build_native(jobs=8, force=True, verbose=False):
         deps = [src] + hdrs + ([os.path.join(CSRC, "fsl_fused.hip")] if s in CONTRACTED else [])
         if force or not _newer(obj, deps):
             flags = [("-ffp-contract=fast" if f == "-ffp-contract=off" else f) for f in HIP_FLAGS] \
-                if s in CONTRACTED else HIP_FLAGS
+                if s in CONTRACTED else list(HIP_FLAGS)
+            if s in SCHED_ILP:
+                flags += ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
             tasks.append([HIPCC] + flags + ["-c", src, "-o", obj])
     for s in HOST_SOURCES + ["fsl_layer2_api.c", "fsl_layer2_main.c", "fsl_layer2_p.c"]:
         src = os.path.join(HOST, s)
