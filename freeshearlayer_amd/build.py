@@ -43,9 +43,12 @@ HIP_SOURCES = ["fsl_api.hip", "fsl_kernels.hip", "fsl_sgs.hip", "fsl_fused.hip",
                "fsl_fused_mixed.hip", "fsl_mlp.hip", "fsl_learned.hip", "fsl_output.hip", "fsl_sgs_march.hip"]
 # the fast build of the FP64 stage kernel (FSL_PREC_FP64_FAST) contracts a * b + c into FMA
 CONTRACTED = {"fsl_fused_fast.hip"}
-# the scheduler's ILP strategy: 0.8 % faster mixed stage kernel and 1.3 % faster network kernel
-# at 384x256x256 (no spills); the FP64 stage kernel spills with it
-SCHED_ILP = {"fsl_fused_mixed.hip", "fsl_mlp.hip"}
+# scheduler strategies per file, from A/B timings at 384x256x256 on one GPU: max-ilp makes the
+# mixed stage kernel 0.8 % and the network kernel 1.3 % faster (no spills; the FP64 stage kernel
+# spills with it), iterative-ilp the FP64 dynamic model 6.5 % and the first FP32 dynamic-model
+# kernel (k_sgs_a<float>) 20 % faster
+SCHED = {"fsl_fused_mixed.hip": "max-ilp", "fsl_mlp.hip": "max-ilp", "fsl_sgs_march.hip": "iterative-ilp",
+         "fsl_sgs.hip": "iterative-ilp"}
 HOST_SOURCES = ["fsl_host.c", "fsl_driver.c", "fsl_stats.c", "fsl_files.c"]
 
 
@@ -85,8 +88,8 @@ def build_native(jobs=8, force=True, verbose=False):
         if force or not _newer(obj, deps):
             flags = [("-ffp-contract=fast" if f == "-ffp-contract=off" else f) for f in HIP_FLAGS] \
                 if s in CONTRACTED else list(HIP_FLAGS)
-            if s in SCHED_ILP:
-                flags += ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+            if s in SCHED:
+                flags += ["-mllvm", "-amdgpu-sched-strategy=" + SCHED[s]]
             tasks.append([HIPCC] + flags + ["-c", src, "-o", obj])
     for s in HOST_SOURCES + ["fsl_layer2_api.c", "fsl_layer2_main.c", "fsl_layer2_p.c"]:
         src = os.path.join(HOST, s)
