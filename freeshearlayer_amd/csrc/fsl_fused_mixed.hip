@@ -134,6 +134,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
     const float dtXf = (float)a.dtX, dtYf = (float)a.dtY, dtZf = (float)a.dtZ;
 
     double cfl_m = 0.0;   // stage 3: CFL maximum over the cells this thread updates
+    // step state of the cell the next plane step updates (stages 2 and 3),
+    // loaded one plane step ahead
+    double bq[5];
     // stage input of the current and the next plane (that of the previous
     // plane, which the update needs, is read back from the plane copy)
     double w1[5], w2[5];
@@ -174,15 +177,12 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
         for (int f = 0; f < 5; f++) w1[f] = w2[f];
         // Loads in the order of first use (the wait counter retires in issue
         // order): next plane's cell and this cell's mu_SGS (end of the
-        // reconstruction), the step state of the updated cell (after the
-        // first barrier), the next plane's outer y/z neighbours (before the
-        // second barrier).
+        // reconstruction), the next plane's outer y/z neighbours (before the
+        // second barrier); the step state of the cell updated in this plane
+        // step was loaded in the previous one.
         load5<PK>(a.in, (long long)min(i + 1, L + 1) * si, pb, fb, colb, w2);
         // own cell's mu_SGS; the faces take theirs from the LDS ring
         const double mu_c = a.ph.dynamic ? bload(plane_rsrc(a.mu, (long long)i * si, pb), colmu) : 0.;
-        // step state of the cell this iteration updates (i - 1): interior lanes only
-        double bq[5];
-        if (STAGE > 1 && interior && i > i0) load5<PK>(a.base, (long long)(i - 1) * si, pb, fb, colb, bq);
         // y neighbours of this plane's cell (written to s_w in the previous
         // iteration); z neighbours from the lanes beside it, and at a row's
         // ends the outer neighbour loaded in the previous iteration
@@ -330,6 +330,10 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused_mixed(MixedArgs a)
 #pragma unroll
             for (int f = 0; f < 5; f++) Fx_prev[f] = Fx[f];
         }
+        // step state of the cell the next plane step updates (i): interior lanes
+        // only; loaded here, after this plane step's update, so that its
+        // latency is covered by a whole plane step
+        if (STAGE > 1 && interior && i >= i0 && i < i1) load5<PK>(a.base, (long long)i * si, pb, fb, colb, bq);
         xp_prev = xp;
         Tprev = Tcur;
         Rprev = Rcur;
