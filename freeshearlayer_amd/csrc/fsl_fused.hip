@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
     const unsigned omy = oc + sjb, omz = oc + 8u;
 
     double cfl_m = 0.0;           // stage 3: CFL maximum over the cells this thread updates
+    double bq[5];                 // step state of the cell the next plane step updates (stages 2, 3)
     double w0[5], w1[5], w2[5];   // stage state at planes i-1, i, i+1 (own column)
     double xp_prev[5];            // + x state of plane i-1
     double Fx_prev[5];
@@ -160,7 +161,8 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
         const long long pl = (long long)i * si;
         // loads in the order of first use: this plane's y neighbours (y
         // reconstruction), the next plane's cell (1/rho velocity), mu_SGS of
-        // this plane, the step state of the cell this iteration updates (i - 1)
+        // this plane (the step state of the cell this plane step updates was
+        // loaded in the previous one)
         const bool inseg = (i >= i0 && i < i1);
         double qyp[5], qym[5];
         if (inseg) {
@@ -176,9 +178,6 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
             if (yface) m_y = bload(mr, omy);
             if (zface) m_z = bload(mr, omz);
         }
-        // step state of the cell this iteration updates (i - 1): interior lanes only
-        double bq[5];
-        if (STAGE > 1 && interior && i > i0) load5<PK>(a.base, (long long)(i - 1) * si, pb, fb, oc, bq);
         lds_barrier();   // the previous plane's LDS readers are done
         const int vc = i & 1, vb = vc ^ 1;   // s_visc slots of planes i and i-1
         {
@@ -272,6 +271,9 @@ __global__ void __launch_bounds__(BJ* BK, MINW) k_fused(FusedArgs a)
         }
 #pragma unroll
         for (int f = 0; f < 5; f++) xp_prev[f] = xp[f];
+        // step state of the cell the next plane step updates (i), interior
+        // lanes: loaded one plane step ahead, after this step's update
+        if (STAGE > 1 && interior && i >= i0 && i < i1) load5<PK>(a.base, (long long)i * si, pb, fb, oc, bq);
 
         // ---- y and z faces of plane i ----
         if (inseg) {
