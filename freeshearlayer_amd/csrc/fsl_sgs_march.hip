@@ -17,8 +17,10 @@
 //               clipped to [0, 0.15], mu_SGS = rho Cd Delta^2 |S|
 // The x pass of a field is kept as running sums: its three terms arrive in
 // three consecutive planes.  Values needed by other columns (u, the y-pass
-// results, u^, A) go through LDS; per-column ones (the product and A
-// accumulators, L, |S|) stay in registers.
+// results, u^, A) go through LDS; per-column and per-point ones (the x sums
+// of u^, the products and A, L, |S|) stay in registers.  The six products
+// and the six components of A each take one y pass and one z pass (seven
+// workgroup barriers per plane).
 //
 // Every value is formed by the operations, in the order, of the reference
 // with real = double (each filter sum ((0 + a h) + b h) + c h; FP contraction
@@ -46,7 +48,8 @@ constexpr int N1 = RJ1 * RK1;                   // 324
 constexpr int NY1 = RJ1 * RK2;                  // y-pass of u^: rows of the one-cell ring, columns of the two-cell ring
 constexpr int NYT = MJ * RK1;                   // y-pass on tile rows, one-cell-ring columns
 static_assert(MJ * MK == kThreads, "one tile column per thread");
-static_assert(NY1 <= 3 * 128 && NYT <= NY1, "y-pass scratch");
+static_assert(3 * NY1 <= 6 * NYT && NYT <= NY1, "y-pass scratch");
+static_assert(N1 <= 2 * kThreads, "at most two ring points per thread");
 
 struct MarchArgs {
     CState5 U;
@@ -69,10 +72,9 @@ __device__ __forceinline__ int pb(int m) { return m < 3 ? m : (m < 5 ? m - 2 : 2
 __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
 {
     __shared__ double s_u[3][3][N2];      // u, v, w of planes P-2, P-1, P (slot P mod 3)
-    __shared__ double s_t1[3][NY1];       // y-pass results of three fields
-    __shared__ double s_hacc[2][3][N1];   // running x sums of u^ (slot by output plane)
+    __shared__ double s_t1[6 * NYT];      // y-pass results: u^ (3 x NY1) or the products / A (6 x NYT)
     __shared__ double s_hu[2][3][N1];     // completed u^ of planes P-1, P-2 (slot by plane parity)
-    __shared__ double s_A[3][N1];         // three components of A on the tile and ring
+    __shared__ double s_A[6][N1];         // the six components of A on the tile and ring
     __shared__ double s_mag[MJ * MK];     // |S| of the tile's cells
 
     const Geo& g = a.g;
@@ -103,11 +105,18 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
     double Lr[6];                // Leonard stress of plane P-2 (completed last iteration)
     double mag_prev = 0.;        // |S| of plane P-2
     double hu3[3];               // u^ of plane P-3 at this column
+    // running x sums of u^ at this thread's ring points e = tid, tid + 256:
+    // hacc2 of plane P-1 (two terms), hacc1 of plane P (one term)
+    double hacc2[2][3], hacc1[2][3];
 #pragma unroll
     for (int m = 0; m < 6; m++) { pacc2[m] = pacc1[m] = aacc2[m] = aacc1[m] = Lr[m] = 0.; }
+#pragma unroll
+    for (int it = 0; it < 2; it++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) hacc2[it][f] = hacc1[it][f] = 0.;
     hu3[0] = hu3[1] = hu3[2] = 0.;
     for (int e = tid; e < 3 * 3 * N2; e += kThreads) (&s_u[0][0][0])[e] = 0.;
-    for (int e = tid; e < 2 * 3 * N1; e += kThreads) { (&s_hacc[0][0][0])[e] = 0.; (&s_hu[0][0][0])[e] = 0.; }
+    for (int e = tid; e < 2 * 3 * N1; e += kThreads) (&s_hu[0][0][0])[e] = 0.;
 
     // The state of the next plane is loaded one iteration ahead (registers:
     // this thread's cells e = tid and tid + 256 of the two-cell ring).  The
@@ -176,7 +185,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
                     t += s_u[s0][f][b + RK2] * h;
                     r = t;
                 }
-                s_t1[f][e] = r;
+                s_t1[f * NY1 + e] = r;
             }
         }
         lds_barrier();
@@ -185,80 +194,81 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
         // value (the x pass starts at i = 1): kept in the running sum of plane
         // 0 at P = 0 and taken as it is at P = 1.
         {
-            const int cA = P & 1, cB = cA ^ 1;          // running sums: plane P-1 (2 terms), plane P (1 term)
             const int hs = (P - 1) & 1;                 // slot of u^ (P-1)
-            for (int e = tid; e < N1; e += kThreads) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                const int e = tid + it * kThreads;
+                if (e >= N1) break;
                 const int rj = e / RK1, rk = e % RK1;
                 const int gj = j0 - 1 + rj, gk = k0 - 1 + rk;
                 const bool ok = pv && gj >= 0 && gj <= H && gk >= 1 && gk <= D;
                 const int b = rj * RK2 + rk + 1;        // (gj, gk) in the y-pass array
 #pragma unroll
                 for (int f = 0; f < 3; f++) {
+                    const double* t1f = s_t1 + f * NY1;
                     double t2 = 0.;
                     if (ok) {
                         double t = 0.;
-                        t += s_t1[f][b - 1] * h;
-                        t += s_t1[f][b] * h;
-                        t += s_t1[f][b + 1] * h;
+                        t += t1f[b - 1] * h;
+                        t += t1f[b] * h;
+                        t += t1f[b + 1] * h;
                         t2 = t;
                     }
                     const double th = t2 * h;
-                    const double acc2 = s_hacc[cA][f][e], acc1 = s_hacc[cB][f][e];
+                    const double acc2 = hacc2[it][f], acc1 = hacc1[it][f];
                     double done = acc2 + th;
                     if (P - 1 == 0) done = acc2;              // u^ (0) = y-pass value
                     else if (P - 1 > L || P - 1 < 0) done = 0.;
                     s_hu[hs][f][e] = done;
                     if (P == 0) {
                         const bool oky = gj >= 1 && gj <= H && gk >= 0 && gk <= D;
-                        s_hacc[cB][f][e] = oky ? s_t1[f][b] : 0.;   // u^ (0), closed at P = 1
+                        hacc2[it][f] = oky ? t1f[b] : 0.;     // u^ (0), closed at P = 1
                     } else {
-                        s_hacc[cB][f][e] = acc1 + th;
+                        hacc2[it][f] = acc1 + th;
                     }
                     double t1 = 0.;
                     t1 += th;
-                    s_hacc[cA][f][e] = t1;                     // 0 + t2 h: first term of plane P+1
+                    hacc1[it][f] = t1;                        // 0 + t2 h: first term of plane P+1
                 }
             }
         }
         // ---- plane P: the six products -- y pass on the tile rows, then z pass
-        // and x sums at this column (two batches of three fields)
+        // and x sums at this column
         double fil[6];   // filtered products of plane P-1 at this column
+        lds_barrier();   // s_t1 free; u^ (P-1) complete
+        for (int e = tid; e < NYT; e += kThreads) {
+            const int rj = e / RK1, rk = e % RK1;
+            const int gj = j0 + rj, gk = k0 - 1 + rk;
+            const bool ok = pv && gj >= 1 && gj <= H && gk >= 0 && gk <= D;
+            const int b = (rj + 2) * RK2 + rk + 1;   // (gj, gk) in the two-cell ring
 #pragma unroll
-        for (int bt = 0; bt < 2; bt++) {
-            lds_barrier();   // s_t1 free
-            for (int e = tid; e < NYT; e += kThreads) {
-                const int rj = e / RK1, rk = e % RK1;
-                const int gj = j0 + rj, gk = k0 - 1 + rk;
-                const bool ok = pv && gj >= 1 && gj <= H && gk >= 0 && gk <= D;
-                const int b = (rj + 2) * RK2 + rk + 1;   // (gj, gk) in the two-cell ring
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    const int m = bt * 3 + q;
-                    double r = 0.;
-                    if (ok) {
-                        const double* ua = s_u[s0][pa(m)];
-                        const double* ub = s_u[s0][pb(m)];
-                        double t = 0.;
-                        t += ua[b - RK2] * ub[b - RK2] * h;
-                        t += ua[b] * ub[b] * h;
-                        t += ua[b + RK2] * ub[b + RK2] * h;
-                        r = t;
-                    }
-                    s_t1[q][e] = r;
+            for (int m = 0; m < 6; m++) {
+                double r = 0.;
+                if (ok) {
+                    const double* ua = s_u[s0][pa(m)];
+                    const double* ub = s_u[s0][pb(m)];
+                    double t = 0.;
+                    t += ua[b - RK2] * ub[b - RK2] * h;
+                    t += ua[b] * ub[b] * h;
+                    t += ua[b + RK2] * ub[b + RK2] * h;
+                    r = t;
                 }
+                s_t1[m * NYT + e] = r;
             }
-            lds_barrier();
+        }
+        lds_barrier();
+        {
             const bool ok = pv && gjc <= H && gkc >= 1 && gkc <= D;
             const int b = cj * RK1 + ck + 1;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int m = bt * 3 + q;
+            for (int m = 0; m < 6; m++) {
+                const double* t1m = s_t1 + m * NYT;
                 double t2 = 0.;
                 if (ok) {
                     double t = 0.;
-                    t += s_t1[q][b - 1] * h;
-                    t += s_t1[q][b] * h;
-                    t += s_t1[q][b + 1] * h;
+                    t += t1m[b - 1] * h;
+                    t += t1m[b] * h;
+                    t += t1m[b + 1] * h;
                     t2 = t;
                 }
                 const double th = t2 * h;
@@ -279,90 +289,81 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
             for (int m = 0; m < 6; m++) Lnew[m] = fil[m] - uf[pa(m)] * uf[pb(m)];
         }
         // ---- plane P-1: strain, |S| and A on the tile and ring; y and z passes
-        // of A, x sums at this column (two batches of three components)
+        // of A, x sums at this column.  (s_A was last read before this plane's
+        // first barrier; s_t1, read by the products' z pass above, is
+        // rewritten after the next barrier.)
         const int PA = P - 1;
         const bool pva = PA >= 0 && PA <= L;
-        double Anext[2][3];   // second batch of this thread's ring cells
         double Af[6];         // A^ of plane P-2 at this column
 #pragma unroll
-        for (int bt = 0; bt < 2; bt++) {
-            lds_barrier();   // s_A and s_t1 free
+        for (int it = 0; it < 2; it++) {
+            const int e = tid + it * kThreads;
+            if (e >= N1) break;
+            const int rj = e / RK1, rk = e % RK1;
+            const int gj = j0 - 1 + rj, gk = k0 - 1 + rk;
+            double A6[6] = {0., 0., 0., 0., 0., 0.};
+            double Sm = 0.;
+            if (PA >= 1 && PA <= L && gj >= 1 && gj <= H && gk >= 1 && gk <= D) {
+                const int b = (rj + 1) * RK2 + rk + 1;   // (gj, gk) in the two-cell ring
+                double gr[3][3];
 #pragma unroll
-            for (int it = 0; it < 2; it++) {
-                const int e = tid + it * kThreads;
-                if (e >= N1) break;
-                if (bt == 1) {
-#pragma unroll
-                    for (int q = 0; q < 3; q++) s_A[q][e] = Anext[it][q];
-                    continue;
+                for (int r = 0; r < 3; r++) {
+                    gr[r][0] = (s_u[s0][r][b] - s_u[s2][r][b]) * r2x;
+                    gr[r][1] = (s_u[s1][r][b + RK2] - s_u[s1][r][b - RK2]) * r2y;
+                    gr[r][2] = (s_u[s1][r][b + 1] - s_u[s1][r][b - 1]) * r2z;
                 }
-                const int rj = e / RK1, rk = e % RK1;
-                const int gj = j0 - 1 + rj, gk = k0 - 1 + rk;
-                double A6[6] = {0., 0., 0., 0., 0., 0.};
-                double Sm = 0.;
-                if (PA >= 1 && PA <= L && gj >= 1 && gj <= H && gk >= 1 && gk <= D) {
-                    const int b = (rj + 1) * RK2 + rk + 1;   // (gj, gk) in the two-cell ring
-                    double gr[3][3];
+                double S[9];
+                S[0] = gr[0][0];
+                S[1] = 0.5 * (gr[0][1] + gr[1][0]);
+                S[2] = 0.5 * (gr[0][2] + gr[2][0]);
+                S[3] = S[1];
+                S[4] = gr[1][1];
+                S[5] = 0.5 * (gr[1][2] + gr[2][1]);
+                S[6] = S[2];
+                S[7] = S[5];
+                S[8] = gr[2][2];
+                Sm = sqrt(2. * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + S[3] * S[3] + S[4] * S[4] +
+                                S[5] * S[5] + S[6] * S[6] + S[7] * S[7] + S[8] * S[8]));
+                const int uniq[6] = {0, 1, 2, 4, 5, 8};
 #pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        gr[r][0] = (s_u[s0][r][b] - s_u[s2][r][b]) * r2x;
-                        gr[r][1] = (s_u[s1][r][b + RK2] - s_u[s1][r][b - RK2]) * r2y;
-                        gr[r][2] = (s_u[s1][r][b + 1] - s_u[s1][r][b - 1]) * r2z;
-                    }
-                    double S[9];
-                    S[0] = gr[0][0];
-                    S[1] = 0.5 * (gr[0][1] + gr[1][0]);
-                    S[2] = 0.5 * (gr[0][2] + gr[2][0]);
-                    S[3] = S[1];
-                    S[4] = gr[1][1];
-                    S[5] = 0.5 * (gr[1][2] + gr[2][1]);
-                    S[6] = S[2];
-                    S[7] = S[5];
-                    S[8] = gr[2][2];
-                    Sm = sqrt(2. * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + S[3] * S[3] + S[4] * S[4] +
-                                    S[5] * S[5] + S[6] * S[6] + S[7] * S[7] + S[8] * S[8]));
-                    const int uniq[6] = {0, 1, 2, 4, 5, 8};
-#pragma unroll
-                    for (int m = 0; m < 6; m++) A6[m] = Sm * S[uniq[m]];
-                }
-                if (rj >= 1 && rj <= MJ && rk >= 1 && rk <= MK) s_mag[(rj - 1) * MK + rk - 1] = Sm;
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    s_A[q][e] = A6[q];
-                    Anext[it][q] = A6[3 + q];
-                }
+                for (int m = 0; m < 6; m++) A6[m] = Sm * S[uniq[m]];
             }
-            lds_barrier();
-            for (int e = tid; e < NYT; e += kThreads) {
-                const int rj = e / RK1, rk = e % RK1;
-                const int gj = j0 + rj, gk = k0 - 1 + rk;
-                const bool ok = pva && gj >= 1 && gj <= H && gk >= 0 && gk <= D;
-                const int b = (rj + 1) * RK1 + rk;   // (gj, gk) in the one-cell ring
+            if (rj >= 1 && rj <= MJ && rk >= 1 && rk <= MK) s_mag[(rj - 1) * MK + rk - 1] = Sm;
 #pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    double r = 0.;
-                    if (ok) {
-                        double t = 0.;
-                        t += s_A[q][b - RK1] * h;
-                        t += s_A[q][b] * h;
-                        t += s_A[q][b + RK1] * h;
-                        r = t;
-                    }
-                    s_t1[q][e] = r;
+            for (int m = 0; m < 6; m++) s_A[m][e] = A6[m];
+        }
+        lds_barrier();
+        for (int e = tid; e < NYT; e += kThreads) {
+            const int rj = e / RK1, rk = e % RK1;
+            const int gj = j0 + rj, gk = k0 - 1 + rk;
+            const bool ok = pva && gj >= 1 && gj <= H && gk >= 0 && gk <= D;
+            const int b = (rj + 1) * RK1 + rk;   // (gj, gk) in the one-cell ring
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                double r = 0.;
+                if (ok) {
+                    double t = 0.;
+                    t += s_A[m][b - RK1] * h;
+                    t += s_A[m][b] * h;
+                    t += s_A[m][b + RK1] * h;
+                    r = t;
                 }
+                s_t1[m * NYT + e] = r;
             }
-            lds_barrier();
+        }
+        lds_barrier();
+        {
             const bool ok = pva && gjc <= H && gkc >= 1 && gkc <= D;
             const int b = cj * RK1 + ck + 1;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int m = bt * 3 + q;
+            for (int m = 0; m < 6; m++) {
+                const double* t1m = s_t1 + m * NYT;
                 double t2 = 0.;
                 if (ok) {
                     double t = 0.;
-                    t += s_t1[q][b - 1] * h;
-                    t += s_t1[q][b] * h;
-                    t += s_t1[q][b + 1] * h;
+                    t += t1m[b - 1] * h;
+                    t += t1m[b] * h;
+                    t += t1m[b + 1] * h;
                     t2 = t;
                 }
                 const double th = t2 * h;
