@@ -19,7 +19,7 @@
 // three consecutive planes.  Values needed by other columns (u, the y-pass
 // results, u^, A) go through LDS; per-column and per-point ones (the x sums
 // of u^, the products and A, L, |S|) stay in registers.  The six products
-// and the six components of A each take one y pass and one z pass (seven
+// and the six components of A each take one y pass and one z pass (six
 // workgroup barriers per plane).
 //
 // Every value is formed by the operations, in the order, of the reference
@@ -145,9 +145,13 @@ __global__ void __launch_bounds__(kThreads, 2) k_sgs_march(MarchArgs a)
     // rho of this column two planes ahead of its use (mu_SGS of plane P-2)
     const long long ocol = (long long)min(gjc, H + 1) * g.sj + min(gkc, D + 1);
     double rho1 = 0., rho2 = 0.;   // planes P-1, P-2
+    lds_barrier();   // the zeroed LDS arrays
 
     for (int P = i0 - 2; P <= i1 + 1; P++) {
-        lds_barrier();   // the previous plane's readers of the slots rewritten below are done
+        // No barrier here: the u slot rewritten below (plane P - 3) was last
+        // read by the previous plane's strain, before its last two barriers;
+        // every other array written before this plane's second barrier was
+        // last read before the previous plane's last barrier.
         const int s0 = mod3(P), s1 = mod3(P - 1), s2 = mod3(P - 2);
         // ---- plane P: u on the tile and two-cell ring (0 outside the ghosted box)
 #pragma unroll
