@@ -443,16 +443,12 @@ void launch_variant(hipStream_t s, int stage, FusedArgs a, int per_cu)
 }
 
 // Occupancy variants of the 16 x 16 tile: 0 = 2 waves/SIMD (default),
-// 1 = 1 wave/SIMD (no register limit below 512)
+// 1 = 1 wave/SIMD (no register limit below 512); read at each launch
 int fused_variant()
 {
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("FSL_FUSED_VARIANT");
-        v = e ? atoi(e) : 0;
-        if (v < 0 || v > 1) v = 0;
-    }
-    return v;
+    const char* e = getenv("FSL_FUSED_VARIANT");
+    const int v = e ? atoi(e) : 0;
+    return (v < 0 || v > 1) ? 0 : v;
 }
 
 }  // namespace

@@ -211,3 +211,37 @@ def test_fast_fp64_hundred_step_divergence():
         for ctx in ctxs:
             ctx.close()
     assert worst[0] <= 1e-12 and worst[-1] <= 1e-10, worst
+
+
+@pytest.mark.parametrize("env", [{"FSL_FP64_FIELD_RSRC": "1"}, {"FSL_FUSED_VARIANT": "1"}],
+                         ids=["field-resources", "one-wave-per-simd"])
+@pytest.mark.parametrize("dynamic", [0, 1])
+def test_other_stage_kernel_forms_match_oracle(monkeypatch, env, dynamic):
+    """The FP64 stage kernel's other compiled forms against the oracle, bit
+    for bit: one buffer resource per field and plane (the form of meshes
+    beyond (4 G + one plane) * 8 >= 2^32 bytes, which FSL_FP64_FIELD_RSRC=1
+    forces at any size) and the one-wave-per-SIMD variant
+    (FSL_FUSED_VARIANT=1)."""
+    for key, val in env.items():
+        monkeypatch.setenv(key, val)
+    orc, ctx = make_pair(30, 29, 15, dynamic=dynamic, seed=31, fused=1)
+    for n in range(3):
+        ctx.step()
+        orc.step()
+        assert ctx.dt == orc.get("dt"), "dt at step %d" % (n + 1)
+        assert_bitwise(orc.state()[:, 1:-1, 1:-1, 1:-1], ctx.download_state()[:, 1:-1, 1:-1, 1:-1],
+                       "interior U after step %d" % (n + 1))
+
+
+def test_fast_build_field_resources_match_its_plane_resources(monkeypatch):
+    """The fast FP64 build in its per-field-resource form gives the bits of
+    its one-resource-per-plane form (the same arithmetic, other addressing)."""
+    out = []
+    for forced in ("0", "1"):
+        monkeypatch.setenv("FSL_FP64_FIELD_RSRC", forced)
+        orc, ctx = make_pair(30, 29, 15, dynamic=1, seed=31, fused=1, precision=fsl.FSL_PREC_FP64_FAST)
+        orc.close()
+        ctx.run(3)
+        out.append(ctx.download_state()[:, 1:-1, 1:-1, 1:-1])
+        ctx.close()
+    assert_bitwise(out[0], out[1], "fast build, per-field vs per-plane resources")
