@@ -1111,11 +1111,8 @@ void launch_mlp_mu(hipStream_t s, CState5 U, const Geo& g, const Phys& ph, doubl
     ma.segL = segL;
     ma.nblocks = nseg * cols;
     const dim3 grid((unsigned)((ma.nblocks + 7) / 8 * 8));   // a multiple of the 8 XCDs
-    static int tp = -1;
-    if (tp < 0) {
-        const char* e = getenv("FSL_MLP_TILES");
-        tp = e && atoi(e) == 2 ? 2 : 1;
-    }
+    const char* e = getenv("FSL_MLP_TILES");   // read at each launch
+    const int tp = e && atoi(e) == 2 ? 2 : 1;
     if (act == kRelu) {
         if (tp == 2) hipLaunchKernelGGL((k_mlp_mu_plane<2, kRelu>), grid, dim3(256), 0, s, ma);
         else hipLaunchKernelGGL((k_mlp_mu_plane<1, kRelu>), grid, dim3(256), 0, s, ma);

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import freeshearlayer_amd as fsl
-from fsl_test_util import random_state
+from fsl_test_util import assert_bitwise, random_state
 import mlp_ref
 
 pytestmark = pytest.mark.gpu
@@ -391,3 +391,28 @@ def test_judge_sets_level_factors(adapt):
     np.testing.assert_allclose(f, expect, rtol=1e-4)
     assert abs(st["scale"] - min(10.0, max(0.1, g))) < 1e-4 * g
     assert np.ptp(f) > 0.05                        # the levels do differ
+
+
+@pytest.mark.parametrize("act", ["tanh", "relu"])
+def test_two_tiles_in_flight_give_the_same_mu(monkeypatch, act):
+    """The network kernel with two 16-cell tiles in flight per wave
+    (FSL_MLP_TILES=2) computes every cell with the same operations as the
+    default one-tile form: mu_SGS agrees bit for bit."""
+    L, H, D = 20, 16, 12
+    case = fsl.case_reference(L, H, D, BL_HIG=5)
+    U = random_state(L, H, D, seed=21)
+    cfg = fsl.config(sgs_model=fsl.FSL_SGS_LEARNED, fused=1)
+    out = []
+    for tiles in ("1", "2"):
+        monkeypatch.setenv("FSL_MLP_TILES", tiles)
+        with fsl.Context(case, cfg) as ctx:
+            ctx.set_inflow(np.linspace(80.0, 190.0, H))
+            ctx.upload_state(U)
+            ctx.mlp_set_thickness(np.full(L + 2, 4.0, np.float32))
+            p, _ = fsl.mlp_init_params(13)
+            ctx.mlp_set_params(p, np.concatenate([np.zeros(8), np.ones(8)]).astype(np.float32))
+            ctx.mlp_set_activation(act)
+            ctx.learned_smagorinsky()
+            out.append(ctx.download_mu())
+    assert_bitwise(out[0], out[1], "mu_SGS, two tiles vs one")
+    assert np.abs(out[0]).max() > 0
