@@ -3,7 +3,7 @@
 Default workload: the FP32 + learned-closure path (mixed precision: FP32
 stencil arithmetic, FP64 state / pressure recovery / RK / CFL; the per-cell
 network for Cs^2 -- the shipped model, ReLU hidden layers -- on the matrix
-cores (FP16 operands, FP32 accumulation; trained in BF16), judged against the dynamic model every 50
+cores (BF16 operands, FP32 accumulation, as in training), judged against the dynamic model every 50
 steps with a dynamic fallback above 10 % error) on the synthetic free shear
 layer at 384x256x256 cells, Mc 0.4 -- tanh mean profile U(y) = Um + dU/2
 tanh(2y/delta_w0), U2/U1 = 0.5, Re 500 on delta_w0 = 4 dy, seeded 2 % dU
@@ -27,7 +27,7 @@ ratio; `fp64_fast_dynamic` the same FP64 leg with the fast build of the stage
 kernel (FMA contraction and reciprocal quotients, not bit-identical).  `roofline` prices the dominant kernel's algorithmic bytes per
 launch against its average launch duration, measured with HIP events on the
 context's stream over the timed steps; `mlp_roofline` the learned-closure
-kernel's algorithmic FLOPs against the FP16 dense peak (its products are FP16).  `traffic` (HBM bytes per
+kernel's algorithmic FLOPs against the BF16 dense peak (its products are BF16).  `traffic` (HBM bytes per
 launch) and the VALU / MFMA busy fractions come from rocprofv3 counter runs
 of the same command (profiles/).  `cpu_baseline` times the reference's
 own serial path (src/, compiled from its sources into
@@ -56,7 +56,7 @@ import freeshearlayer_amd as fsl  # noqa: E402
 from freeshearlayer_amd import shear_case  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-F16_PEAK_TFLOPS = 2500.0   # MI355X dense FP16 / BF16 MFMA
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 MODEL = os.path.join(ROOT, "freeshearlayer_amd", "data", "sgs_mlp.bin")
 METRIC = "Mcell-updates/s per RK stage on 1 MI355X, FP32+learned SGS vs FP64 dyn-Smag"
 UNIT = "Mcell-updates/s per RK stage"
@@ -342,8 +342,8 @@ def main(argv=None):
         ms, cnt = leg["prof"]["mlp_mu"]
         tf = L * H * D * MLP_FLOP_PER_CELL / (ms / cnt / 1e3) / 1e12
         mtraffic, valu, mfma, prov = counter_figures("mlp_mu", a.precision, wl)
-        mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": F16_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": round(tf / F16_PEAK_TFLOPS, 5),
+        mlp_roofline = {"bound": "mfma", "kernel": "mlp_mu", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
                         "flop_per_cell": MLP_FLOP_PER_CELL, "mfma_flop_per_cell": MLP_MFMA_FLOP_PER_CELL,
                         "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": int(cnt),
                         "traffic": mtraffic, "alg_bytes_per_launch": 40 * L * H * D,
@@ -396,7 +396,7 @@ def main(argv=None):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(leg["t"] / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision.startswith("fp64") else "f32+f64+f16" if a.sgs == "learned" else "f32+f64",
+        "dtype": "f64" if a.precision.startswith("fp64") else "f32+f64+bf16" if a.sgs == "learned" else "f32+f64",
         "data": "synthetic shear layer (tanh profile, Mc %.1f, U2/U1 0.5, Re 500, 2%% seeded perturbations, "
                 "acoustic CFL %.2f)" % (a.mach, a.cfl),
         "config": {"workload": wl, "mesh": [L, H, D], "stages": 3, "sgs": a.sgs, "precision": a.precision,

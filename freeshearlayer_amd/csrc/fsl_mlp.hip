@@ -42,12 +42,12 @@ constexpr int PB1 = 0;                  // W1: 4 n-tiles
 constexpr int PB2 = PB1 + 4 * 64;       // W2: 4 k-steps x 4 n-tiles
 constexpr int PB2T = PB2 + 16 * 64;     // W2 transposed (backward): 4 k-steps x 4 n-tiles
 constexpr int PB3 = PB2T + 16 * 64;     // w3 in column 0: 4 k-steps
-// FP16 operands of the inference kernels (K = 2 log2 e, the tanh argument
+// BF16 operands of the inference kernels (K = 2 log2 e, the tanh argument
 // scale; the hidden layers carry r = 1 / (2^u + 1), tanh = 1 - 2 r):
 constexpr int PH1 = PB3 + 4 * 64;       // K W1
 constexpr int PH2 = PH1 + 4 * 64;       // -2 K W2, 16x16x32 operands (below)
 constexpr int PH3 = PH2 + 16 * 64;      // -2 w3 in column 0, 16x16x32 operands
-// Layers 2 and 3 use v_mfma_f32_16x16x32_f16: the operand of K slab s (input
+// Layers 2 and 3 use v_mfma_f32_16x16x32_bf16: the operand of K slab s (input
 // units 32 s .. 32 s + 31), output tile t and lane l is the pair of 16x16x16
 // operands of K tiles 2 s and 2 s + 1 (units 32 s + 4 (l / 16) + e and
 // 32 s + 16 + 4 (l / 16) + e), 16 bytes: the A and B elements an MFMA pairs
@@ -63,40 +63,41 @@ typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+// Operands of the inference products: BF16 (as in training), FP32
+// accumulation.
+typedef __bf16 iv4 __attribute__((ext_vector_type(4)));
+typedef __bf16 iv2 __attribute__((ext_vector_type(2)));
+typedef short s4v __attribute__((ext_vector_type(4)));
 
-// FP32 -> FP16, round to nearest even, two values per v_cvt_pk_f16_f32
-__device__ __forceinline__ unsigned cvt2h(float a, float b)
+// FP32 -> BF16, round to nearest even, two values per v_cvt_pk_bf16_f32
+__device__ __forceinline__ unsigned cvt2i(float a, float b)
 {
-    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, h2));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, iv2));
 }
 
-__device__ __forceinline__ h4 cvt4h(float a, float b, float c, float d)
+__device__ __forceinline__ iv4 cvt4i(float a, float b, float c, float d)
 {
-    return __builtin_bit_cast(h4, (u2){cvt2h(a, b), cvt2h(c, d)});
+    return __builtin_bit_cast(iv4, (u2){cvt2i(a, b), cvt2i(c, d)});
 }
 
-__device__ __forceinline__ unsigned short f2h(float f) { return __builtin_bit_cast(unsigned short, (_Float16)f); }
-
-// normalised features enter the FP16 products clamped to the FP16 range
-// (beyond a few units the first layer is saturated anyway)
-constexpr float kF16Max = 60000.f;
-__device__ __forceinline__ float clamp_h(float x) { return __builtin_amdgcn_fmed3f(x, -kF16Max, kF16Max); }
+__device__ __forceinline__ unsigned short f2i(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ f4 mfma_h32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ h8 cat8(h4 a, h4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
-// 16x16x32 operand (K slab s, tile t) of lane l in a packed region (two 8-byte units)
-__device__ __forceinline__ h8 pack8(const bf4* region, int s, int t, int ntiles, int lane)
+__device__ __forceinline__ f4 mfma_i(iv4 a, iv4 b, f4 c)
 {
-    return *reinterpret_cast<const h8*>(region + 2 * ((s * ntiles + t) * 64 + lane));
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(bf4, a), __builtin_bit_cast(bf4, b), c, 0, 0, 0);
 }
-__device__ __forceinline__ h4 as_h4(bf4 v) { return __builtin_bit_cast(h4, v); }
+typedef __bf16 iv8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4 mfma_i32(iv8 a, iv8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ iv8 cat8(iv4 a, iv4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+// 16x16x32 operand (K slab s, tile t) of lane l in a packed region (two 8-byte units)
+__device__ __forceinline__ iv8 pack8(const bf4* region, int s, int t, int ntiles, int lane)
+{
+    return *reinterpret_cast<const iv8*>(region + 2 * ((s * ntiles + t) * 64 + lane));
+}
+__device__ __forceinline__ iv4 as_i4(bf4 v) { return __builtin_bit_cast(iv4, v); }
 
 // tanh(z + b) = 1 - 2 / (2^(K (z + b)) + 1), K = 2 log2(e), with kb = K b:
 // one FMA for the scaled argument, v_exp_f32, v_rcp_f32 and one FMA.  No
@@ -111,24 +112,32 @@ __device__ __forceinline__ float act(float z, float kb)
 
 // Inference activation r = 1 / (2^u + 1) of the scaled argument
 // u = K (z + b) (tanh(z + b) = 1 - 2 r): v_exp_f32, an add, v_rcp_f32.  The
-// inference kernels multiply FP16 operands: the features, K W1 onto K b1;
+// inference kernels multiply BF16 operands: the features, K W1 onto K b1;
 // r1 and -2 K W2 onto K (b2 + sum_k W2[k][.]); r2 and -2 w3 onto
-// b3 + sum_k w3[k] -- the network's products with h = 1 - 2 r.  FP16 keeps
-// r to 2^-12 (h to 5e-4), finer than the BF16 h of training (2e-3 near 1).
+// b3 + sum_k w3[k] -- the network's products with h = 1 - 2 r.  BF16 keeps
+// r to 2^-9 relative: h to 2e-3 near +1 (as the BF16 h of training) and to
+// 4e-3 near -1.
 __device__ __forceinline__ float act_r(float u)
 {
     return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.f);
 }
 
 // ReLU networks (ACT = FSL_MLP_RELU): h = max(z + b, 0).  The inference form
-// multiplies FP16 operands W1, W2, w3 onto accumulators that start at b1,
-// b2, b3, and takes the maximum after the conversion to FP16 (max and the
-// rounding commute): one v_pk_max_f16 per two units.
+// multiplies BF16 operands W1, W2, w3 onto accumulators that start at b1,
+// b2, b3, and takes the maximum after the conversion to BF16 (max and the
+// rounding commute), so its hidden values are those of the training form
+// up to the accumulation order.
 constexpr int kTanh = 0, kRelu = 1;
-__device__ __forceinline__ h4 relu4(h4 v) { return __builtin_elementwise_max(v, (h4){0, 0, 0, 0}); }
+// max(x, 0) of four BF16 values as a signed 16-bit maximum of their bit
+// patterns (a negative BF16 value, -0 included, is a negative integer): one
+// v_pk_max_i16 per two units
+__device__ __forceinline__ iv4 relu4(iv4 v)
+{
+    return __builtin_bit_cast(iv4, __builtin_elementwise_max(__builtin_bit_cast(s4v, v), (s4v){0, 0, 0, 0}));
+}
 
-// FP16 bits of element e of v (through the packed 32-bit words)
-__device__ __forceinline__ unsigned short h4_bits(h4 v, int e)
+// BF16 bits of element e of v (through the packed 32-bit words)
+__device__ __forceinline__ unsigned short i4_bits(iv4 v, int e)
 {
     const u2 w = __builtin_bit_cast(u2, v);
     return (unsigned short)((e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xffffu));
@@ -159,10 +168,10 @@ __device__ __forceinline__ float infer_b1(float b)
 
 // four hidden values of the inference form from an output tile's accumulators
 template <int ACT>
-__device__ __forceinline__ h4 act4(const f4 z)
+__device__ __forceinline__ iv4 act4(const f4 z)
 {
-    if (ACT == kRelu) return relu4(cvt4h(z[0], z[1], z[2], z[3]));
-    return cvt4h(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
+    if (ACT == kRelu) return relu4(cvt4i(z[0], z[1], z[2], z[3]));
+    return cvt4i(act_r(z[0]), act_r(z[1]), act_r(z[2]), act_r(z[3]));
 }
 
 // Each wavefront of the inference kernel stages its own 64 cells' features in
@@ -198,9 +207,9 @@ __device__ __forceinline__ bf4 lds_row4(const unsigned short* p)
     return v;
 }
 
-// Forward pass of 64 samples held in ws.X (already normalised; BF16, FP16
-// for INFER); leaves H1, H2 in LDS (BF16 h; for INFER FP16 r of tanh
-// networks, FP16 h of ReLU networks) and the outputs in ws.y.  Training
+// Forward pass of 64 samples held in ws.X (already normalised, BF16);
+// leaves H1, H2 in LDS (BF16 h; for INFER BF16 r of tanh networks, BF16 h
+// of ReLU networks) and the outputs in ws.y.  Training
 // (INFER false) uses the BF16 tiles; the inference form matches
 // k_mlp_mu_tr's arithmetic.  Caller syncs.
 template <bool INFER, int ACT>
@@ -214,13 +223,13 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         for (int tn = 0; tn < 4; tn++) {
             const float b = w.master[OB1 + tn * 16 + r16], kb = infer_b1<ACT>(b);
             f4 acc;
-            if (INFER) acc = mfma_h(as_h4(a), as_h4(w.pack[PH1 + tn * 64 + lane]), f4{kb, kb, kb, kb});
+            if (INFER) acc = mfma_i(as_i4(a), as_i4(w.pack[PH1 + tn * 64 + lane]), f4{kb, kb, kb, kb});
             else acc = mfma(a, w.pack[PB1 + tn * 64 + lane], f4{0.f, 0.f, 0.f, 0.f});
-            const h4 hi = act4<ACT>(acc);
+            const iv4 hi = act4<ACT>(acc);
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 ws.H1[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
-                    INFER ? h4_bits(hi, e) : f2bf(act_train<ACT>(acc[e], b));
+                    INFER ? i4_bits(hi, e) : f2bf(act_train<ACT>(acc[e], b));
         }
     }
     __syncthreads();
@@ -233,9 +242,9 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         }
         if (INFER) {
             for (int sl = 0; sl < 2; sl++) {
-                const h8 a = cat8(as_h4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
-                                  as_h4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
-                for (int tn = 0; tn < 4; tn++) acc[tn] = mfma_h32(a, pack8(w.pack + PH2, sl, tn, 4, lane), acc[tn]);
+                const iv8 a = cat8(as_i4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
+                                  as_i4(lds_row4(&ws.H1[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
+                for (int tn = 0; tn < 4; tn++) acc[tn] = mfma_i32(a, pack8(w.pack + PH2, sl, tn, 4, lane), acc[tn]);
             }
         } else {
             for (int ks = 0; ks < 4; ks++) {
@@ -245,11 +254,11 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         }
         for (int tn = 0; tn < 4; tn++) {
             const float b = w.master[OB2 + tn * 16 + r16];
-            const h4 hi = act4<ACT>(acc[tn]);
+            const iv4 hi = act4<ACT>(acc[tn]);
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 ws.H2[(tm * 16 + 4 * q4 + e) * NH + tn * 16 + r16] =
-                    INFER ? h4_bits(hi, e) : f2bf(act_train<ACT>(acc[tn][e], b));
+                    INFER ? i4_bits(hi, e) : f2bf(act_train<ACT>(acc[tn][e], b));
         }
     }
     __syncthreads();
@@ -258,9 +267,9 @@ __device__ __forceinline__ void forward64(WaveLds& ws, const MlpW& w, int lane)
         f4 acc = {0.f, 0.f, 0.f, 0.f};
         if (INFER) {
             for (int sl = 0; sl < 2; sl++) {
-                const h8 a = cat8(as_h4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
-                                  as_h4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
-                acc = mfma_h32(a, pack8(w.pack + PH3, sl, 0, 1, lane), acc);
+                const iv8 a = cat8(as_i4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 4 * q4])),
+                                  as_i4(lds_row4(&ws.H2[(tm * 16 + r16) * NH + 32 * sl + 16 + 4 * q4])));
+                acc = mfma_i32(a, pack8(w.pack + PH3, sl, 0, 1, lane), acc);
             }
         } else {
             for (int ks = 0; ks < 4; ks++) {
@@ -327,12 +336,12 @@ __device__ __forceinline__ void stencil_gradient(const Stencil& s, const Phys& p
 #pragma unroll
     for (int e = 0; e < 3; e++) {
         const float h = (float)ph.r2del[e];
-        const float h2 = (float)(ph.rdel[e] * ph.rdel[e]);
+        const float iv2 = (float)(ph.rdel[e] * ph.rdel[e]);
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             const float up = s.up[e][r], um = s.um[e][r];
             gr[r][e] = (up - um) * h;
-            lap[r] += (up - 2.f * s.uc[r] + um) * h2;
+            lap[r] += (up - 2.f * s.uc[r] + um) * iv2;
         }
     }
 }
@@ -360,7 +369,7 @@ __device__ __forceinline__ float strain_sq(const float gr[3][3])
 __device__ __forceinline__ float cell_features(const Stencil& st, const FeatArgs& a, int i, int j, float f[NIN])
 {
     // FP32 with FMA contraction and the hardware square root (1 ulp): the
-    // features reach the network rounded to FP16 (BF16 in training)
+    // features reach the network rounded to BF16
 #pragma clang fp contract(fast)
     const Geo& g = a.g;
     float gr[3][3], lap[3];
@@ -449,7 +458,7 @@ __global__ void __launch_bounds__(256) k_mlp_features(FeatArgs a, const float* c
 // cells as columns (N).  The output tile D[unit][cell] then sits in the lanes
 // exactly as the next layer's B operand B[unit][cell] (lane = cell + 16
 // (unit / 4), element = unit % 4), so activations go from the accumulators
-// through the activation (act4) and FP16 rounding straight into the next
+// through the activation (act4) and BF16 rounding straight into the next
 // product.  The weight tiles are the A operands (same lane layout as the
 // packed B tiles) and stay in registers for the whole march; only the 8
 // features of each cell pass through LDS (one 8-byte read per lane and
@@ -489,13 +498,13 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
     const MlpW& w = ma.w;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q4 = lane >> 4;
-    h4 A1[4];
-    h8 A2[2][4], A3[2];
+    iv4 A1[4];
+    iv8 A2[2][4], A3[2];
     float kb1[4][4], kb2[4][4];
     const float* cf = reinterpret_cast<const float*>(w.pack + PHF);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        A1[t] = as_h4(w.pack[PH1 + t * 64 + lane]);
+        A1[t] = as_i4(w.pack[PH1 + t * 64 + lane]);
         A2[0][t] = pack8(w.pack + PH2, 0, t, 4, lane);
         A2[1][t] = pack8(w.pack + PH2, 1, t, 4, lane);
 #pragma unroll
@@ -583,18 +592,18 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
     };
     double qo[4], qh[4] = {1., 0., 0., 0.};
     {
-        double a0[4], a1[4], a2[4], h0[4] = {1., 0., 0., 0.}, h1[4] = {1., 0., 0., 0.}, h2[4] = {1., 0., 0., 0.};
+        double a0[4], a1[4], a2[4], h0[4] = {1., 0., 0., 0.}, h1[4] = {1., 0., 0., 0.}, iv2[4] = {1., 0., 0., 0.};
         ld4(i0 - 1, oo, a0);
         ld4(i0, oo, a1);
         ld4(i0 + 1, oo, a2);
         if (hal) {
             ld4(i0 - 1, oh, h0);
             ld4(i0, oh, h1);
-            ld4(i0 + 1, oh, h2);
+            ld4(i0 + 1, oh, iv2);
         }
         put(i0 - 1, a0, h0);
         put(i0, a1, h1);
-        put(i0 + 1, a2, h2);
+        put(i0 + 1, a2, iv2);
     }
     ld4(i0 + 2, oo, qo);
     if (hal) ld4(i0 + 2, oh, qh);
@@ -632,26 +641,26 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
         if (hal) ld4(i + 3, oh, qh);
         float fn[NIN];
 #pragma unroll
-        for (int q = 0; q < NIN; q++) fn[q] = clamp_h((f[q] - nm[q]) * ns[q]);
-        *(h4*)&X[lane * NIN] = cvt4h(fn[0], fn[1], fn[2], fn[3]);
-        *(h4*)&X[lane * NIN + 4] = cvt4h(fn[4], fn[5], fn[6], fn[7]);
+        for (int q = 0; q < NIN; q++) fn[q] = (f[q] - nm[q]) * ns[q];
+        *(iv4*)&X[lane * NIN] = cvt4i(fn[0], fn[1], fn[2], fn[3]);
+        *(iv4*)&X[lane * NIN + 4] = cvt4i(fn[4], fn[5], fn[6], fn[7]);
         wave_lds_sync();
         float ymine = 0.f;
 #pragma unroll
         for (int s0 = 0; s0 < 4; s0 += TP) {
             // B operand of layer 1: features 4 q4 .. 4 q4 + 3 of cell st * 16 + r16 (K padded to 16)
-            h4 xb[TP];
+            iv4 xb[TP];
 #pragma unroll
             for (int u = 0; u < TP; u++) {
-                xb[u] = h4{0, 0, 0, 0};
-                if (q4 < 2) xb[u] = *(const h4*)&X[((s0 + u) * 16 + r16) * NIN + 4 * q4];
+                xb[u] = iv4{0, 0, 0, 0};
+                if (q4 < 2) xb[u] = *(const iv4*)&X[((s0 + u) * 16 + r16) * NIN + 4 * q4];
             }
-            h4 h1[TP][4], h2[TP][4];
+            iv4 h1[TP][4], iv2[TP][4];
 #pragma unroll
             for (int tt = 0; tt < 4; tt++)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
-                    const f4 z = mfma_h(A1[tt], xb[u], f4{kb1[tt][0], kb1[tt][1], kb1[tt][2], kb1[tt][3]});
+                    const f4 z = mfma_i(A1[tt], xb[u], f4{kb1[tt][0], kb1[tt][1], kb1[tt][2], kb1[tt][3]});
                     h1[u][tt] = act4<ACT>(z);
                 }
 #pragma unroll
@@ -659,15 +668,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
 #pragma unroll
                 for (int u = 0; u < TP; u++) {
                     f4 z = {kb2[tt][0], kb2[tt][1], kb2[tt][2], kb2[tt][3]};
-                    z = mfma_h32(A2[0][tt], cat8(h1[u][0], h1[u][1]), z);
-                    z = mfma_h32(A2[1][tt], cat8(h1[u][2], h1[u][3]), z);
-                    h2[u][tt] = act4<ACT>(z);
+                    z = mfma_i32(A2[0][tt], cat8(h1[u][0], h1[u][1]), z);
+                    z = mfma_i32(A2[1][tt], cat8(h1[u][2], h1[u][3]), z);
+                    iv2[u][tt] = act4<ACT>(z);
                 }
 #pragma unroll
             for (int u = 0; u < TP; u++) {
                 f4 z = {0.f, 0.f, 0.f, 0.f};
-                z = mfma_h32(A3[0], cat8(h2[u][0], h2[u][1]), z);
-                z = mfma_h32(A3[1], cat8(h2[u][2], h2[u][3]), z);
+                z = mfma_i32(A3[0], cat8(iv2[u][0], iv2[u][1]), z);
+                z = mfma_i32(A3[1], cat8(iv2[u][2], iv2[u][3]), z);
                 // row 0 of the output tile: y of cell st * 16 + r16 sits in lane r16, element 0
                 const float yt = __shfl(z[0], r16, 64);
                 if (q4 == s0 + u) ymine = yt + b3;
@@ -692,7 +701,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
     WaveLds& ws = lds[wv];
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = s < n ? f2h(clamp_h((X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q])) : 0;
+    for (int q = 0; q < NIN; q++) ws.X[lane * KP1 + q] = s < n ? f2i((X[s * NIN + q] - w.norm[q]) * w.norm[NIN + q]) : 0;
 #pragma unroll
     for (int q = NIN; q < KP1; q++) ws.X[lane * KP1 + q] = 0;
     __syncthreads();
@@ -850,7 +859,7 @@ __global__ void k_mlp_adam(AdamArgs a)
     a.grad[p] = 0.f;
 }
 
-// BF16 B operands in lane order from the FP32 master weights, and the FP16
+// BF16 B operands in lane order from the FP32 master weights, and the BF16
 // operands and FP32 constants of the inference form of activation act
 __global__ void k_mlp_pack(const float* master, bf4* pack, int act)
 {
@@ -897,15 +906,15 @@ __global__ void k_mlp_pack(const float* master, bf4* pack, int act)
         } else if (tile < PH1 / 64) {               // w3 in column 0
             const int ks = tile - PB3 / 64;
             x = r16 == 0 ? master[OW3 + ks * 16 + kk] : 0.f;
-        } else if (tile < PH2 / 64) {               // K W1 (FP16; W1 for ReLU)
+        } else if (tile < PH2 / 64) {               // K W1 (W1 for ReLU)
             const int n = (tile - PH1 / 64) * 16 + r16;
             x = kk < NIN ? (relu ? 1.f : kTanhK) * master[OW1 + kk * NH + n] : 0.f;
-        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (FP16; W2), 16x16x32 order
+        } else if (tile < PH3 / 64) {               // -2 K W2 [k][n] (W2 for ReLU), 16x16x32 order
             const int u = t - PH2, pos = u >> 1, half = u & 1;
             const int sl = pos / 256, tn = (pos / 64) % 4, ln = pos % 64;
             const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e, nn = tn * 16 + (ln & 15);
             x = (relu ? 1.f : -2.f * kTanhK) * master[OW2 + (ks * 16 + kq) * NH + nn];
-        } else {                                    // -2 w3 in column 0 (FP16; w3), 16x16x32 order
+        } else {                                    // -2 w3 in column 0 (w3 for ReLU), 16x16x32 order
             const int u = t - PH3, pos = u >> 1, half = u & 1;
             const int sl = pos / 64, ln = pos % 64;
             const int ks = 2 * sl + half, kq = 4 * (ln >> 4) + e;
@@ -914,7 +923,7 @@ __global__ void k_mlp_pack(const float* master, bf4* pack, int act)
         v[e] = x;
     }
     if (t >= PH1) {
-        pack[t] = __builtin_bit_cast(bf4, cvt4h(v[0], v[1], v[2], v[3]));
+        pack[t] = __builtin_bit_cast(bf4, cvt4i(v[0], v[1], v[2], v[3]));
         return;
     }
     bf4 o;

@@ -348,7 +348,7 @@ int fsl_favre_write_profiles(const char *dir, unsigned HIG, const double *ru);
 /* --- learned SGS closure (sgs_model = FSL_SGS_LEARNED) --- */
 /* Network 8 -> 64 -> 64 -> 1 with tanh or ReLU hidden layers
    (fsl_mlp_set_activation); BF16 operands on the matrix cores with FP32
-   accumulation for training, FP16 operands for inference.  Output y gives Cd = clip(0.15 y, 0,
+   accumulation for training and inference.  Output y gives Cd = clip(0.15 y, 0,
    0.15) and mu_SGS = rho Cd Delta^2 |S| (src/turbulence.c:436-445 with the
    network's Cd in place of the Germano-Lilly one).  Features per interior
    cell, from central differences of u = (rho u)/rho (FP32):
@@ -368,7 +368,7 @@ int fsl_favre_write_profiles(const char *dir, unsigned HIG, const double *ru);
 /* host: Glorot-uniform weights (splitmix64 of seed), zero biases, identity norm */
 void fsl_mlp_init_params(unsigned long long seed, float *params, float *norm);
 /* Hidden-layer activation: tanh (the default) or ReLU h = max(z, 0).  The
-   inference of a ReLU network takes one packed FP16 maximum per two hidden
+   inference of a ReLU network takes one packed 16-bit maximum per two hidden
    units where tanh takes v_exp_f32, v_rcp_f32 and an add per unit. */
 #define FSL_MLP_TANH 0
 #define FSL_MLP_RELU 1

@@ -33,31 +33,19 @@ def normalise(X, norm):
 
 TANH_K = np.float32(2.8853900817779268)   # 2 log2(e): tanh(z) = 1 - 2 / (2^(K z) + 1)
 
-def f16(x):
-    """Round to FP16 (nearest even), returned as FP32."""
-    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
-
-
 def forward(X, params, norm, exact=False, inference=False, act="tanh"):
     """Network output for raw features X (n x 8).  exact=True skips every
     BF16 rounding (FP32 network).  inference=True follows the inference
-    kernels instead: FP16 operands, features clamped to +-60000, hidden values
-    r = 1 / (2^u + 1) of the scaled arguments u (tanh = 1 - 2 r): layer 1
-    FP16(x) FP16(K W1) + K b1, layer 2 FP16(r1) FP16(-2 K W2) +
-    K (b2 + sum_k W2[k]), output FP16(r2) FP16(-2 w3) + b3 + sum_k w3[k];
-    training uses BF16(x), BF16(W) and tanh.  act="relu": h = max(z, 0),
-    inference with FP16(x) FP16(W1) + b1, FP16(h1) FP16(W2) + b2 and
-    FP16(h2) FP16(w3) + b3, the hidden values rounded to FP16."""
+    kernels instead: BF16 operands, hidden values r = 1 / (2^u + 1) of the
+    scaled arguments u (tanh = 1 - 2 r): layer 1 BF16(x) BF16(K W1) + K b1,
+    layer 2 BF16(r1) BF16(-2 K W2) + K (b2 + sum_k W2[k]), output
+    BF16(r2) BF16(-2 w3) + b3 + sum_k w3[k]; training uses BF16(x), BF16(W)
+    and tanh.  act="relu": h = max(z, 0) rounded to BF16, the same steps in
+    training and inference."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
     xn = (np.asarray(X, np.float32) - norm[:8]) * norm[8:]
     if act == "relu":
         relu = lambda z: np.maximum(z, 0.0)
-        if inference and not exact:
-            x = f16(np.clip(xn, -60000.0, 60000.0))
-            h1 = f16(relu(x.astype(np.float64) @ f16(W1) + b1))
-            h2 = f16(relu(h1.astype(np.float64) @ f16(W2) + b2))
-            y = h2.astype(np.float64) @ f16(w3) + b3
-            return y.astype(np.float32), (x, h1, h2)
         r = (lambda a: np.asarray(a, np.float32)) if exact else bf16
         x = r(xn)
         h1 = r(relu(x.astype(np.float64) @ r(W1) + b1).astype(np.float32))
@@ -65,13 +53,13 @@ def forward(X, params, norm, exact=False, inference=False, act="tanh"):
         y = h2.astype(np.float64) @ r(w3) + b3
         return y.astype(np.float32), (x, h1, h2)
     if inference and not exact:
-        x = f16(np.clip(xn, -60000.0, 60000.0))
-        u1 = x.astype(np.float64) @ f16(TANH_K * W1) + TANH_K * b1
-        r1 = f16(1.0 / (np.exp2(u1) + 1.0))
+        x = bf16(xn)
+        u1 = x.astype(np.float64) @ bf16(TANH_K * W1) + TANH_K * b1
+        r1 = bf16((1.0 / (np.exp2(u1) + 1.0)).astype(np.float32))
         c2 = TANH_K * (b2 + W2.astype(np.float32).sum(0, dtype=np.float32))
-        u2 = r1.astype(np.float64) @ f16(np.float32(-2.0) * TANH_K * W2) + c2
-        r2 = f16(1.0 / (np.exp2(u2) + 1.0))
-        y = r2.astype(np.float64) @ f16(np.float32(-2.0) * w3) + (b3 + w3.sum(dtype=np.float32))
+        u2 = r1.astype(np.float64) @ bf16(np.float32(-2.0) * TANH_K * W2) + c2
+        r2 = bf16((1.0 / (np.exp2(u2) + 1.0)).astype(np.float32))
+        y = r2.astype(np.float64) @ bf16(np.float32(-2.0) * w3) + (b3 + w3.sum(dtype=np.float32))
         return y.astype(np.float32), (x, 1.0 - 2.0 * r1, 1.0 - 2.0 * r2)
     r = (lambda a: np.asarray(a, np.float32)) if exact else bf16
     x = r(xn)

@@ -5,9 +5,9 @@ learned mu_SGS, training, and the judge / fallback logic of the step.
 
 Tolerances: the MFMA product of BF16-exact operands differs from FP64 only
 by FP32 accumulation (rtol 1e-5).  Network outputs differ from the numpy
-restatement of the inference kernels' FP16 arithmetic where the device
-exp2 / rcp (1 ulp) and numpy round a hidden value to different FP16
-neighbours: max |dy| <= 2e-2.  Features are FP32 on the device and
+restatement of the inference kernels' BF16 arithmetic where the device
+exp2 / rcp (1 ulp) or the accumulation order and numpy round a hidden value
+to different BF16 neighbours: max |dy| <= 2e-2.  Features are FP32 on the device and
 FP64 in numpy: errors within 2e-4 of each feature's scale, and within 1e-4
 relative for 99 % of cells.  The fallback path must reproduce the dynamic model bit for bit."""
 import numpy as np
