@@ -52,31 +52,41 @@ __global__ void __launch_bounds__(kBlock) k_ghost_fill(State5 U, Geo g, Bnd b)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int L = g.L, H = g.H, D = g.D;
-    const long long px = (long long)(H + 2) * (D + 2), py = (long long)L * (D + 2), pz = (long long)L * H;
+    // each section's size is below 2^32 (a mesh holds at most 4e9 cells), so
+    // the divisions within a section are 32-bit
+    const unsigned W = (unsigned)(D + 2);
+    const long long px = (long long)(H + 2) * W, py = (long long)L * W, pz = (long long)L * H;
     int i, j, k;
     if (t < 2 * px) {
-        const int side = (int)(t / px);
-        const long long r = t - side * px;
+        const bool side = t >= px;
+        const unsigned r = (unsigned)(side ? t - px : t), q = r / W;
         i = side ? L + 1 : 0;
-        j = (int)(r / (D + 2));
-        k = (int)(r % (D + 2));
+        j = (int)q;
+        k = (int)(r - q * W);
     } else if ((t -= 2 * px) < 2 * py) {
-        const int side = (int)(t / py);
-        const long long r = t - side * py;
+        const bool side = t >= py;
+        const unsigned r = (unsigned)(side ? t - py : t), q = r / W;
         j = side ? H + 1 : 0;
-        i = 1 + (int)(r / (D + 2));
-        k = (int)(r % (D + 2));
+        i = 1 + (int)q;
+        k = (int)(r - q * W);
     } else if ((t -= 2 * py) < pz) {
         // z periodic ghosts of one interior row (i, j): k = 0 <- D and k = D+1 <- 1.
-        // One thread per row touches the row's first and last cache lines once.
-        i = 1 + (int)(t / H);
-        j = 1 + (int)(t % H);
+        // One thread per row touches the row's first and last cache lines once;
+        // all ten loads are issued before the stores (one memory round trip).
+        const unsigned r = (unsigned)t, q = r / (unsigned)H;
+        i = 1 + (int)q;
+        j = 1 + (int)(r - q * (unsigned)H);
         const long long row = cidx(g, i, j, 0);
+        double lo[5], hi[5];
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-            const double lo = U.f[f][row + D], hi = U.f[f][row + 1];
-            U.f[f][row] = lo;
-            U.f[f][row + D + 1] = hi;
+            lo[f] = U.f[f][row + D];
+            hi[f] = U.f[f][row + 1];
+        }
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            U.f[f][row] = lo[f];
+            U.f[f][row + D + 1] = hi[f];
         }
         return;
     } else {
