@@ -469,6 +469,21 @@ constexpr int MFP = MW * MW;             // 324 ring points
 constexpr int MHALO = MFP - MT * MT;     // 68 of them outside the tile
 constexpr int MRING = 4;                 // planes in the LDS ring
 static_assert(MT * MT == 256 && MHALO <= 256 && (MRING & (MRING - 1)) == 0, "mu kernel tiling");
+// LDS layout of one field of one ring plane: columns kk = 1 .. MT of every
+// ring row jj as rows of MT floats (a wave reads four consecutive rows at
+// one offset: each fills its own quarter of the 64 banks), and the side
+// columns kk = 0 and MW - 1 in blocks of 64 floats at the one bank of the
+// row's quarter that a z-neighbour read leaves free (kk = MW - 1: the
+// quarter's first bank, read with columns 2 .. MT; kk = 0: its last,
+// read with columns 1 .. MT - 1), so no read of the stencil has a bank
+// conflict.
+constexpr int MFL = MW * MT + (MW + 3) / 4 * 64;   // 608 floats
+__device__ __forceinline__ int ring_pos(int jj, int kk)
+{
+    if (kk >= 1 && kk <= MT) return jj * MT + kk - 1;
+    const int side = MW * MT + (jj >> 2) * 64 + 16 * (jj & 3);
+    return kk == 0 ? side + 15 : side;
+}
 
 constexpr int kRsrcWord3 = 0x00020000;   // gfx9 buffer resource, untyped access
 
@@ -491,7 +506,7 @@ struct MuArgs {
 template <int TP, int ACT>   // 16-cell tiles in flight per layer; hidden activation
 __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
 {
-    __shared__ float s_u[MRING][3][MFP];   // velocity of the tile and ring, plane p in slot p % MRING
+    __shared__ float s_u[MRING][3][MFL];   // velocity of the tile and ring, plane p in slot p % MRING
     __shared__ double s_rho[MRING][MT * MT];   // density of the tile's cells (mu_SGS = rho Cd Delta^2 |S|)
     __shared__ unsigned short sX[4][64 * NIN];
     const FeatArgs& a = ma.f;
@@ -544,7 +559,9 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
     const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
     const int j = j0 + tr, k = k0 + tc;
     const bool valid = j <= H && k <= D;
-    const int po = (tr + 1) * MW + tc + 1;   // own point in the ring
+    const int po = ring_pos(tr + 1, tc + 1);   // own point in the ring
+    const int pyp = ring_pos(tr + 2, tc + 1), pym = ring_pos(tr, tc + 1);   // y neighbours
+    const int pzp = ring_pos(tr + 1, tc + 2), pzm = ring_pos(tr + 1, tc);   // z neighbours
     // ring point outside the tile of threads t < MHALO: rows 0 and MW - 1,
     // then columns 0 and MW - 1 of rows 1 .. MT
     int jj, kk;
@@ -562,7 +579,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
         kk = MW - 1;
     }
     const bool hal = t < MHALO;
-    const int ph = hal ? jj * MW + kk : po;
+    const int ph = hal ? ring_pos(jj, kk) : po;
     // in-plane byte offsets (rows and columns beyond the ghost layer clamped:
     // only invalid lanes read those points)
     const unsigned sjb = (unsigned)g.sj * 8u;
@@ -620,10 +637,10 @@ __global__ void __launch_bounds__(256, 3) k_mlp_mu_plane(MuArgs ma)
                 st.uc[r] = s_u[sc][r][po];
                 st.up[0][r] = s_u[sp][r][po];
                 st.um[0][r] = s_u[sm][r][po];
-                st.up[1][r] = s_u[sc][r][po + MW];
-                st.um[1][r] = s_u[sc][r][po - MW];
-                st.up[2][r] = s_u[sc][r][po + 1];
-                st.um[2][r] = s_u[sc][r][po - 1];
+                st.up[1][r] = s_u[sc][r][pyp];
+                st.um[1][r] = s_u[sc][r][pym];
+                st.up[2][r] = s_u[sc][r][pzp];
+                st.um[2][r] = s_u[sc][r][pzm];
             }
         }
         float f[NIN], smag = 0.f;
