@@ -145,11 +145,23 @@ const SgsAvg* sgs_avg_of(fsl_ctx* c)
     return &a;
 }
 
+// The dynamic model of every context is the FP64 marching kernel, mixed
+// contexts included: at 384x256x256 it takes 1.36 ms where the FP32
+// two-kernel form (filters and Germano contraction in FP32, 16 float scratch
+// arrays) takes 2.34 ms, and it is the parity arithmetic.  FSL_SGS_FP32=1
+// selects the FP32 form in mixed contexts (read at each call).
+bool sgs_fp32_form(const fsl_ctx* c)
+{
+    if (c->cfg.precision != FSL_PREC_MIXED) return false;
+    const char* e = getenv("FSL_SGS_FP32");
+    return e && e[0] == '1' && e[1] == 0;
+}
+
 int ensure_sgs(fsl_ctx* c)
 {
-    // the FP64 model keeps its intermediates on chip; the FP32 model (mixed
-    // contexts) has 16 float scratch arrays
-    if (c->sgs || c->cfg.precision != FSL_PREC_MIXED) return FSL_OK;
+    // the FP64 model keeps its intermediates on chip; the FP32 form has 16
+    // float scratch arrays, allocated on its first use
+    if (c->sgs || !sgs_fp32_form(c)) return FSL_OK;
     const size_t n = sgs_f32_scratch_bytes(c->g);
     HIPCHK(c, hipMalloc(&c->sgs, n));
     // the A tensor's ghost layer must read as 0 (src/turbulence.c:369-379
@@ -581,7 +593,7 @@ int fsl_dynamic_smagorinsky(fsl_ctx* c)
 {
     int rc = ensure_sgs(c);
     if (rc) return rc;
-    if (c->cfg.precision == FSL_PREC_MIXED)
+    if (sgs_fp32_form(c))
         launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, nullptr,
                        sgs_avg_of(c));
     else

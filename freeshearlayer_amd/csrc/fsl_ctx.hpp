@@ -115,6 +115,8 @@ void prof_mark(void* arg, int id, int end);
 State5 st5(double* base, long long G);
 CState5 cst5(const double* base, long long G);
 int ensure_sgs(fsl_ctx* c);
+// the FP32 two-kernel form of the dynamic model runs (mixed contexts with FSL_SGS_FP32=1)
+bool sgs_fp32_form(const fsl_ctx* c);
 // the averaging of the dynamic model in force (null: pointwise)
 const SgsAvg* sgs_avg_of(fsl_ctx* c);
 int check_launch(fsl_ctx* c, const char* what);

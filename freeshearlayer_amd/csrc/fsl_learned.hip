@@ -192,7 +192,7 @@ int dynamic_mu(fsl_ctx* c, float* cd)
 {
     int rc = ensure_sgs(c);
     if (rc) return rc;
-    if (c->cfg.precision == FSL_PREC_MIXED)
+    if (sgs_fp32_form(c))
         launch_sgs_f32(c->stream, cst5(c->U, c->g.G), c->mu, c->sgs, c->g, c->ph, prof_mark, c, cd, sgs_avg_of(c));
     else
         launch_sgs(c->stream, cst5(c->U, c->g.G), c->mu, c->g, c->ph, prof_mark, c, cd, sgs_avg_of(c));

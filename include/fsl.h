@@ -154,7 +154,8 @@ typedef struct fsl_ctx fsl_ctx;
 #define FSL_SGS_AVG_XZ   2
 
 #define FSL_PREC_FP64    0   /* parity path: FP64, no contraction, reference op order */
-#define FSL_PREC_MIXED   1   /* FP32 stencils and SGS, FP64 state / RK / p recovery / CFL */
+#define FSL_PREC_MIXED   1   /* FP32 stencils and SGS fluxes, FP64 state / RK / p recovery /
+                                CFL / dynamic model */
 #define FSL_PREC_FP64_FAST 2 /* FP64 with the fused stage kernel built with FMA contraction
                                 and reciprocal quotients (needs fused = 1); within 1e-12
                                 relative of the parity path after one step, not bit-identical;
@@ -233,9 +234,10 @@ int fsl_rk_update(fsl_ctx *ctx, int scheme, int stage);
 /* src/helpers.c:23-52: device max-reduction, then deltaT *= maxCoNum / maxCo
    (convective) or deltaT = cfl / max(...) (acoustic).  maxCo may be NULL. */
 int fsl_check_co_num(fsl_ctx *ctx, double *maxCo);
-/* src/turbulence.c:103-507 on the step state U -> mu_SGS (FP64 contexts:
-   the reference's operations; mixed-precision contexts: the same model with
-   FP32 filters, contraction and scratch) */
+/* src/turbulence.c:103-507 on the step state U -> mu_SGS: the reference's
+   FP64 operations in every context (mixed-precision contexts with the
+   environment variable FSL_SGS_FP32=1: the same model with FP32 filters,
+   contraction and scratch) */
 int fsl_dynamic_smagorinsky(fsl_ctx *ctx);
 /* Averaging of the dynamic coefficient, FSL_SGS_AVG_* (no reference
    counterpart: the reference's model is pointwise; default NONE).  The

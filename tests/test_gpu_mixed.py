@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import freeshearlayer_amd as fsl
+from fsl_test_util import assert_bitwise
 
 pytestmark = pytest.mark.gpu
 
@@ -60,12 +61,19 @@ def test_mixed_reference_case_runs_and_stays_close():
     assert max(e) < 1e-3
 
 
-def test_fp32_dynamic_model_vs_fp64():
-    """The mixed path's dynamic model (FP32 filters, Germano contraction and
-    scratch, velocities shifted per tile so the Leonard stress does not
-    cancel) against the FP64 one on the same developed state: bounds on the
-    bulk and the worst cell relative to the largest mu_SGS, and on the total
-    (the dissipation level the closure sets)."""
+@pytest.mark.parametrize("fp32_form", [True, False])
+def test_fp32_dynamic_model_vs_fp64(fp32_form, monkeypatch):
+    """The FP32 form of the dynamic model (FSL_SGS_FP32=1 in a mixed context:
+    FP32 filters, Germano contraction and scratch, velocities shifted per
+    tile so the Leonard stress does not cancel) against the FP64 one on the
+    same developed state: bounds on the bulk and the worst cell relative to
+    the largest mu_SGS, and on the total (the dissipation level the closure
+    sets).  Without the variable a mixed context runs the FP64 model itself:
+    bit for bit the FP64 context's mu_SGS."""
+    if fp32_form:
+        monkeypatch.setenv("FSL_SGS_FP32", "1")
+    else:
+        monkeypatch.delenv("FSL_SGS_FP32", raising=False)
     case = fsl.case_reference(48, 32, 24)
     U, u_in, mu = fsl.ic_shear_layer(case)
     case.mu_L = mu
@@ -86,6 +94,9 @@ def test_fp32_dynamic_model_vs_fp64():
         ctx.ghost_fill()
         ctx.dynamic_smagorinsky()
         m32 = ctx.download_mu()
+    if not fp32_form:
+        assert_bitwise(m64, m32, "mixed context's dynamic mu_SGS")
+        return
     I = (slice(1, -1),) * 3
     a, b = m32[I], m64[I]
     scale = np.abs(b).max()

@@ -32,8 +32,15 @@ def averaged_mu(lm, mm, fac, mode):
 
 
 @pytest.mark.parametrize("mode", [fsl.FSL_SGS_AVG_Z, fsl.FSL_SGS_AVG_XZ])
-@pytest.mark.parametrize("precision", [fsl.FSL_PREC_FP64, fsl.FSL_PREC_MIXED])
-def test_averaged_coefficient_matches_oracle_terms(mode, precision):
+@pytest.mark.parametrize("precision,fp32_form", [(fsl.FSL_PREC_FP64, False), (fsl.FSL_PREC_MIXED, False),
+                                                (fsl.FSL_PREC_MIXED, True)])
+def test_averaged_coefficient_matches_oracle_terms(mode, precision, fp32_form, monkeypatch):
+    # a mixed context runs the FP64 model unless FSL_SGS_FP32=1 selects the FP32 form
+    if fp32_form:
+        monkeypatch.setenv("FSL_SGS_FP32", "1")
+    else:
+        monkeypatch.delenv("FSL_SGS_FP32", raising=False)
+    exact = not fp32_form
     L, H, D = 20, 18, 24          # tiles that do not divide the mesh
     orc, ctx = make_pair(L, H, D, dynamic=1, seed=31, precision=precision)
     with ctx:
@@ -48,7 +55,7 @@ def test_averaged_coefficient_matches_oracle_terms(mode, precision):
         mu = ctx.download_mu()
         err = np.abs(mu - ref).max() / np.abs(ref).max()
         print("mode %d precision %d: max |mu - ref| / max mu = %.2e" % (mode, precision, err))
-        if precision == fsl.FSL_PREC_FP64:
+        if exact:
             np.testing.assert_allclose(mu, ref, rtol=1e-10, atol=1e-14 * np.abs(ref).max())
         else:
             assert err < 2e-4
@@ -56,12 +63,12 @@ def test_averaged_coefficient_matches_oracle_terms(mode, precision):
         I = (slice(1, -1),) * 3
         c = np.divide(mu[I], fac[I], out=np.zeros_like(mu[I]), where=fac[I] > 0)
         spread = np.ptp(c, axis=2 if mode == fsl.FSL_SGS_AVG_Z else (0, 2))
-        assert spread.max() < (1e-12 if precision == fsl.FSL_PREC_FP64 else 1e-5)
+        assert spread.max() < (1e-12 if exact else 1e-5)
         # back to the reference's pointwise model
         ctx.set_sgs_average(fsl.FSL_SGS_AVG_NONE)
         ctx.dynamic_smagorinsky()
         orc.dynamic()
-        if precision == fsl.FSL_PREC_FP64:
+        if exact:
             assert_bitwise(orc.cell(10), ctx.download_mu(), "pointwise mu_SGS after averaging")
 
 
